@@ -1,0 +1,167 @@
+/*
+ * oa_ortho.h -- C ABI of the MI355X orthonormalization library
+ * (openatom_amd/liboa_ortho.so).
+ *
+ * Two layers:
+ *
+ *  1. BLAS drop-ins with the exact Fortran ABI the reference's path calls
+ *     (dgemm_).  A Charm++ build links this library ahead of src_mathlib (the
+ *     reference already links with -Wl,--allow-multiple-definition,
+ *     makefiles/config.mk:32), and every DGEMM on the path runs on the GPU.
+ *     Host or device pointers are accepted; the call is synchronous.
+ *
+ *  2. Path-level entry points (oa_*) over device-resident buffers, on the
+ *     handle's HIP stream.  They replace the PairCalculator forward/backward
+ *     DGEMM tuples and the Ortho S->T loop with single calls.
+ *
+ * Layouts (both layers' path calls): a block of n states is row-major
+ * [n][ld] doubles holding 2*npw interleaved (re, im) coefficients per row --
+ * the collated layout the reference hands to DGEMM (MessageDataCollator.h:222,
+ * ckPairCalculator.C:871-872).  n x n matrices (S, T, Lambda) are row-major
+ * [x][y] with leading dimension n (ckPairCalculator.C:889: outData[x*nY + y]).
+ *
+ * Status codes: 0 success; > 0 the 1-based index of an invalid argument
+ * (BLAS INFO convention, dgemm.f:169-194); < 0 one of OA_ERR_*.
+ * Every entry point is reentrant: no global mutable state other than the
+ * per-thread default handle used by dgemm_.
+ */
+#ifndef OA_ORTHO_H
+#define OA_ORTHO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OA_SUCCESS 0
+#define OA_ERR_HIP (-1)           /* a HIP runtime call failed               */
+#define OA_ERR_NOT_CONVERGED (-2) /* S->T reached maxit (ortho.C:129-133)    */
+#define OA_ERR_NO_DEVICE (-3)     /* no usable gfx950 device                 */
+#define OA_ERR_UNSUPPORTED (-4)   /* precision/variant not available         */
+#define OA_ERR_CAPACITY (-5)      /* request exceeds a documented limit      */
+
+/* Arithmetic of the O(n^2 npw) products (overlap, Lambda, back-multiply).
+ * S->T, the G=0 term and all reductions stay FP64 in every mode.
+ *   OA_PREC_FP64      FP64 MFMA (v_mfma_f64_16x16x4), the drop-in form.
+ *   OA_PREC_INT8      Ozaki digit slices on int8 MFMA, exact int32 sums.
+ *   OA_PREC_FP16      Ozaki digit slices on fp16 MFMA, exact fp32 sums.
+ *   OA_PREC_BF16      Ozaki digit slices on bf16 MFMA, exact fp32 sums.
+ *   OA_PREC_FP32      Ozaki digit slices on fp32 MFMA, exact fp32 sums.
+ * For the slice modes, OA_PREC_SLICES(p, s) selects s digit slices
+ * (2 <= s <= 6); the default is 3. */
+#define OA_PREC_FP64 0
+#define OA_PREC_INT8 1
+#define OA_PREC_FP16 2
+#define OA_PREC_BF16 3
+#define OA_PREC_FP32 4
+#define OA_PREC_SLICES(p, s) ((p) | ((s) << 8))
+
+typedef struct oa_context *oa_handle_t;
+
+/* Device-side S->T report, written by the asynchronous S->T entry points. */
+typedef struct oa_ns_report {
+  int32_t iterations; /* iterations performed                          */
+  int32_t done;       /* 1 once the loop has stopped                   */
+  int32_t status;     /* OA_SUCCESS or OA_ERR_NOT_CONVERGED            */
+  int32_t pad;
+  double error;       /* sqrt(sum (T_old - T_new)^2 / n^2), last iterate */
+} oa_ns_report;
+
+/* ---- handles ---------------------------------------------------------- */
+
+/* Create a handle on HIP device `device`, with its own non-blocking stream.
+ * No counterpart in the reference (a Charm++ PE's BLAS has no handle). */
+int oa_create(oa_handle_t *h, int device);
+int oa_destroy(oa_handle_t h);
+/* Replace the handle's stream (a hipStream_t); NULL restores the handle's own. */
+int oa_set_stream(oa_handle_t h, void *stream);
+void *oa_get_stream(oa_handle_t h);
+/* Block until the handle's stream is idle. */
+int oa_synchronize(oa_handle_t h);
+/* Pre-size the handle's workspace for instances up to (n, npw) so that later
+ * calls allocate nothing (needed before hipGraph capture). */
+int oa_reserve(oa_handle_t h, int n, int npw);
+const char *oa_status_string(int status);
+
+/* ---- BLAS drop-ins (Fortran ABI) --------------------------------------- */
+
+/* Replaces the reference's DGEMM (src_mathlib/dgemm.f:1-313) as declared at
+ * ckPairCalculator.h:26-48 / CLA_Matrix.C:9-17 and called by
+ * PairCalculator::multiplyForward (ckPairCalculator.C:889),
+ * bwMultiplyHelper (:1835, :1842, :1879, :1929), bwMultiplyDynOrthoT (:2018),
+ * CLA_Matrix::multiply (CLA_Matrix.C:556) and genmatmul (genmatmul.f:33).
+ * Column-major, FP64 MFMA on the GPU.  Invalid arguments return silently with
+ * C untouched, as the reference's empty XERBLA does (xerbla.f). */
+void dgemm_(const char *transa, const char *transb, const int *m, const int *n,
+            const int *k, const double *alpha, const double *a, const int *lda,
+            const double *b, const int *ldb, const double *beta, double *c,
+            const int *ldc);
+
+/* Same operation on device pointers, asynchronous on the handle's stream;
+ * returns the BLAS INFO value instead of discarding it. */
+int oa_dgemm(oa_handle_t h, char transa, char transb, int m, int n, int k,
+             double alpha, const double *a, int lda, const double *b, int ldb,
+             double beta, double *c, int ldc);
+
+/* ---- path level (device pointers, asynchronous on the handle's stream) -- */
+
+/* Symmetric PairCalculator forward: S = alpha * sum_k w_k psi_x[k] psi_y[k]
+ * with w_k = 1/2 on the leading 2*nkx0 doubles (the kx=0 points that
+ * CP_State_GSpacePlane.C:3025-3028 scales by 1/sqrt2 before the alpha=2
+ * DGEMM) and 1 elsewhere.  Replaces multiplyForward + contributeSubTiles +
+ * sumMatrixDouble (ckPairCalculator.C:821-1016, :20-53).  Only upper-triangle
+ * tiles are computed; S is written in full (exactly symmetric).  psi rows have
+ * leading dimension ldpsi >= 2*npw.  If maxdev != NULL it receives
+ * max_ij |S_ij - diag*delta_ij| (Ortho::array_diag_max, ortho.h:282-313). */
+int oa_overlap_sym(oa_handle_t h, int n, int npw, const double *psi,
+                   long ldpsi, int nkx0, double alpha, int prec, double *S,
+                   double diag, double *maxdev);
+
+/* Asymmetric PairCalculator forward (the Lambda instance, alpha = 1 in the
+ * reference, cpaimd.C:2510-2516): L[x][y] = alpha * sum_k psi_x[k] F_y[k]
+ * over all 2*npw doubles.  Replaces ckPairCalculator.C:889 for the asymmetric
+ * instance. */
+int oa_lambda(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
+              const double *F, long ldF, double alpha, int prec, double *L);
+
+/* Ortho S -> T (ortho.C:153-971): B = S/2 (nspin == 1) or S (nspin == 2,
+ * lsda), A = I; C = 3I - A*B; B = B*C/2; A = C*A/2 until
+ * sqrt(sum (A_old - A_new)^2 / n^2) <= tol or the iteration count reaches
+ * maxit.  Returns OA_ERR_NOT_CONVERGED when the count reached maxit at exit
+ * (the reference aborts there, ortho.C:129-133); T then holds the last
+ * iterate.  Synchronous: iters / err are host pointers (may be NULL). */
+int oa_s_to_t(oa_handle_t h, int n, const double *S, int nspin, double tol,
+              int maxit, double *T, int *iters, double *err);
+
+/* Asynchronous S->T: fixed launch sequence of maxit guarded iterations that
+ * become no-ops once converged (capturable in a hipGraph).  The report is a
+ * device pointer. */
+int oa_s_to_t_async(oa_handle_t h, int n, const double *S, int nspin,
+                    double tol, int maxit, double *T, oa_ns_report *report);
+
+/* Backward PairCalculator multiply (bwMultiplyHelper, ckPairCalculator.C:
+ * 1835 symmetric, :1842 asymmetric; bwMultiplyDynOrthoT :2018 with
+ * alpha = -1): out[y] = alpha * sum_x M[y][x] psi[x] + beta * out[y]
+ * (transM == 0), or M[x][y] (transM != 0), over 2*npw doubles per row. */
+int oa_backmul(oa_handle_t h, int n, int npw, const double *M, int transM,
+               const double *psi, long ldpsi, double alpha, double beta,
+               double *out, long ldout, int prec);
+
+/* One symmetric orthonormalisation step -- overlap, S->T, T*psi -- as the
+ * reference's GSP -> PC -> Ortho -> PC -> GSP round trip performs it
+ * (call stack: gspace.ci:158 -> CP_State_GSpacePlane.C:2975 ->
+ * ckPairCalculator.C:821 -> ortho.C:153..971 -> ckPairCalculator.C:1201 ->
+ * CP_State_GSpacePlane.C:3232).  S, T (n*n) and psi_out ([n][ldout]) are
+ * device outputs; psi_out must not alias psi.  Synchronous when iters/err
+ * are non-NULL host pointers, else asynchronous. */
+int oa_ortho_step(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
+                  int nkx0, int nspin, double tol, int maxit, int prec,
+                  double *S, double *T, double *psi_out, long ldout,
+                  int *iters, double *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OA_ORTHO_H */

@@ -1,0 +1,126 @@
+"""ctypes binding of liboa_ortho.so (the C ABI declared in include/oa_ortho.h).
+
+The shared library is the product: every product call goes through it.  If it
+is not built, loading raises -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboa_ortho.so")
+HEADER = os.path.join(HERE, "..", "include", "oa_ortho.h")
+
+OA_SUCCESS = 0
+OA_ERR_HIP = -1
+OA_ERR_NOT_CONVERGED = -2
+OA_ERR_NO_DEVICE = -3
+OA_ERR_UNSUPPORTED = -4
+OA_ERR_CAPACITY = -5
+
+OA_PREC_FP64 = 0
+OA_PREC_INT8 = 1
+OA_PREC_FP16 = 2
+OA_PREC_BF16 = 3
+OA_PREC_FP32 = 4
+
+
+def prec_code(prec) -> int:
+    """'fp64' | 'int8x5' | 'fp16x3' | 'bf16x4' | 'fp32x3' | int -> C precision code."""
+    if isinstance(prec, int):
+        return prec
+    p = prec.lower()
+    if p == "fp64":
+        return OA_PREC_FP64
+    m = re.fullmatch(r"(int8|fp16|bf16|fp32)(?:x(\d))?", p)
+    if not m:
+        raise ValueError(f"unknown precision {prec!r}")
+    base = {"int8": OA_PREC_INT8, "fp16": OA_PREC_FP16, "bf16": OA_PREC_BF16,
+            "fp32": OA_PREC_FP32}[m.group(1)]
+    slices = int(m.group(2) or 3)
+    return base | (slices << 8)
+
+
+class OaError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} (status {status})")
+
+
+class OrthoNotConverged(OaError):
+    """S->T hit invsqr_max_iter (the reference aborts, ortho.C:129-133)."""
+
+
+class oa_ns_report(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("done", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("error", ctypes.c_double)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_l = ctypes.c_long
+_d = ctypes.c_double
+
+
+def _sig(f, res, *args):
+    f.restype = res
+    f.argtypes = list(args)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built: run `python -m openatom_amd.build` "
+            "(the HIP library is required; there is no CPU path)")
+    L = ctypes.CDLL(LIB_PATH)
+    _sig(L.oa_create, _i, ctypes.POINTER(_vp), _i)
+    _sig(L.oa_destroy, _i, _vp)
+    _sig(L.oa_set_stream, _i, _vp, _vp)
+    _sig(L.oa_get_stream, _vp, _vp)
+    _sig(L.oa_synchronize, _i, _vp)
+    _sig(L.oa_reserve, _i, _vp, _i, _i)
+    _sig(L.oa_status_string, ctypes.c_char_p, _i)
+    _sig(L.oa_dgemm, _i, _vp, ctypes.c_char, ctypes.c_char, _i, _i, _i, _d, _vp, _i, _vp,
+         _i, _d, _vp, _i)
+    _sig(L.oa_overlap_sym, _i, _vp, _i, _i, _vp, _l, _i, _d, _i, _vp, _d, _vp)
+    _sig(L.oa_lambda, _i, _vp, _i, _i, _vp, _l, _vp, _l, _d, _i, _vp)
+    _sig(L.oa_s_to_t, _i, _vp, _i, _vp, _i, _d, _i, _vp, ctypes.POINTER(_i),
+         ctypes.POINTER(_d))
+    _sig(L.oa_s_to_t_async, _i, _vp, _i, _vp, _i, _d, _i, _vp, _vp)
+    _sig(L.oa_backmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
+    _sig(L.oa_ortho_step, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp, _l,
+         ctypes.POINTER(_i), ctypes.POINTER(_d))
+    _lib = L
+    return L
+
+
+def status_string(st: int) -> str:
+    try:
+        return lib().oa_status_string(st).decode()
+    except ImportError:
+        return str(st)
+
+
+def check(st: int, what: str) -> None:
+    if st == OA_SUCCESS:
+        return
+    if st == OA_ERR_NOT_CONVERGED:
+        raise OrthoNotConverged(st, what)
+    if st > 0:
+        raise ValueError(f"{what}: invalid argument #{st}")
+    raise OaError(st, what)
+
+
+def declared_symbols() -> list[str]:
+    """Every function the public header declares (for the export test)."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(\w+)\s*\(", txt, flags=re.M)
+    return sorted({n for n in names if n not in ("if", "defined")})

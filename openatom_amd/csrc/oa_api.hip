@@ -1,0 +1,447 @@
+// C ABI of liboa_ortho.so: handles, the dgemm_ drop-in and the path-level
+// entry points.  Every arithmetic operation runs in a HIP kernel on the
+// device; there is no host compute path.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "oa_internal.h"
+
+void oa_note_hip_error(hipError_t e, const char *what, const char *file, int line) {
+  fprintf(stderr, "[oa_ortho] HIP error '%s' in %s (%s:%d)\n", hipGetErrorString(e), what,
+          file, line);
+}
+
+namespace {
+
+// Makes the handle's device current for the scope of an API call and
+// restores the caller's device afterwards.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+bool is_capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) return false;
+  return cs != hipStreamCaptureStatusNone;
+}
+
+bool lsame(char a, char b) {
+  if (a >= 'a' && a <= 'z') a = (char)(a - 'a' + 'A');
+  return a == b;
+}
+
+// dgemm.f:161-194 argument checks; returns INFO.
+int dgemm_info(char ta, char tb, int m, int n, int k, int lda, int ldb, int ldc) {
+  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+  const int nrowa = nota ? m : k, nrowb = notb ? k : n;
+  if (!nota && !lsame(ta, 'C') && !lsame(ta, 'T')) return 1;
+  if (!notb && !lsame(tb, 'C') && !lsame(tb, 'T')) return 2;
+  if (m < 0) return 3;
+  if (n < 0) return 4;
+  if (k < 0) return 5;
+  if (lda < (nrowa > 1 ? nrowa : 1)) return 8;
+  if (ldb < (nrowb > 1 ? nrowb : 1)) return 10;
+  if (ldc < (m > 1 ? m : 1)) return 13;
+  return 0;
+}
+
+// Column-major DGEMM on device pointers, expressed as the row-major problem
+// C^T[n][m] = alpha * op(B)^T * op(A)^T + beta * C^T.
+int enqueue_dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha,
+                  const double *a, int lda, const double *b, int ldb, double beta, double *c,
+                  int ldc) {
+  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+  oa_gemm_problem p;
+  p.M = n;
+  p.N = m;
+  p.K = (alpha == 0.0) ? 0 : k;  // alpha == 0: C = beta*C without touching A, B
+  p.X = b;
+  p.ldx = ldb;
+  p.Y = a;
+  p.ldy = lda;
+  p.C = c;
+  p.ldc = ldc;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.diag_add = 0.0;
+  return oa_launch_gemm_f64(s, !notb, !nota, &p, 1, nullptr);
+}
+
+bool is_device_ptr(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+thread_local oa_context *tl_default = nullptr;
+
+oa_context *default_handle() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  if (tl_default && tl_default->device == dev) return tl_default;
+  oa_handle_t h = nullptr;
+  if (oa_create(&h, dev) != OA_SUCCESS) return nullptr;
+  tl_default = h;  // the previous default (other device) stays alive for reuse-free exit
+  return h;
+}
+
+size_t span_elems(int ld, int rows, int cols) {
+  if (rows <= 0 || cols <= 0) return 0;
+  return (size_t)ld * (size_t)(cols - 1) + (size_t)rows;
+}
+
+}  // namespace
+
+int oa_ws_require(oa_context *c, size_t bytes) {
+  if (bytes <= c->ws.bytes) return OA_SUCCESS;
+  if (is_capturing(c->stream)) return OA_ERR_CAPACITY;  // no allocation under capture
+  OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (c->ws.ptr) OA_HIP_CHECK(hipFree(c->ws.ptr));
+  c->ws.ptr = nullptr;
+  c->ws.bytes = 0;
+  const size_t gran = (size_t)1 << 24;
+  const size_t sz = (bytes + gran - 1) / gran * gran;
+  OA_HIP_CHECK(hipMalloc(&c->ws.ptr, sz));
+  c->ws.bytes = sz;
+  return OA_SUCCESS;
+}
+
+extern "C" {
+
+const char *oa_status_string(int st) {
+  switch (st) {
+    case OA_SUCCESS: return "success";
+    case OA_ERR_HIP: return "HIP runtime error";
+    case OA_ERR_NOT_CONVERGED: return "S->T iteration limit reached (invsqr_max_iter)";
+    case OA_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case OA_ERR_UNSUPPORTED: return "unsupported precision or variant";
+    case OA_ERR_CAPACITY: return "request exceeds a limit (workspace under capture, size)";
+    default: return st > 0 ? "invalid argument (BLAS INFO index)" : "unknown status";
+  }
+}
+
+int oa_create(oa_handle_t *out, int device) {
+  if (!out) return 1;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return OA_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return OA_ERR_NO_DEVICE;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    fprintf(stderr, "[oa_ortho] device %d is %s; this library is built for gfx950 only\n",
+            device, prop.gcnArchName);
+    return OA_ERR_NO_DEVICE;
+  }
+  DeviceGuard g(device);
+  if (!g.ok) return OA_ERR_NO_DEVICE;
+  oa_context *c = new oa_context();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->report, sizeof(oa_ns_report)) != hipSuccess ||
+      hipHostMalloc(&c->host_report, sizeof(oa_ns_report), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    oa_destroy(c);
+    return OA_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return OA_SUCCESS;
+}
+
+int oa_destroy(oa_handle_t c) {
+  if (!c) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+  if (c->stream && c->stream != c->own_stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ws.ptr) (void)hipFree(c->ws.ptr);
+  if (c->report) (void)hipFree(c->report);
+  if (c->host_report) (void)hipHostFree(c->host_report);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (tl_default == c) tl_default = nullptr;
+  delete c;
+  return OA_SUCCESS;
+}
+
+int oa_set_stream(oa_handle_t c, void *stream) {
+  if (!c) return 1;
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return OA_SUCCESS;
+}
+
+void *oa_get_stream(oa_handle_t c) { return c ? (void *)c->stream : nullptr; }
+
+int oa_synchronize(oa_handle_t c) {
+  if (!c) return 1;
+  DeviceGuard g(c->device);
+  OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return OA_SUCCESS;
+}
+
+static size_t ws_for(int n, int npw) {
+  const int K = 2 * npw;
+  size_t b = oa_overlap_f64_ws_bytes(n, K);
+  const size_t l = oa_ntsplit_f64_ws_bytes(n, n, K);
+  if (l > b) b = l;
+  const size_t ns = oa_ns_ws_bytes(n);
+  if (ns > b) b = ns;
+  return b;
+}
+
+int oa_reserve(oa_handle_t c, int n, int npw) {
+  if (!c) return 1;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  DeviceGuard g(c->device);
+  return oa_ws_require(c, ws_for(n, npw));
+}
+
+// ------------------------------------------------------------- BLAS drop-ins
+
+int oa_dgemm(oa_handle_t c, char ta, char tb, int m, int n, int k, double alpha,
+             const double *a, int lda, const double *b, int ldb, double beta, double *cc,
+             int ldc) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  const int info = dgemm_info(ta, tb, m, n, k, lda, ldb, ldc);
+  if (info) return info;
+  if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  return enqueue_dgemm(c->stream, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
+}
+
+void dgemm_(const char *transa, const char *transb, const int *pm, const int *pn, const int *pk,
+            const double *palpha, const double *a, const int *plda, const double *b,
+            const int *pldb, const double *pbeta, double *c, const int *pldc) {
+  const char ta = *transa, tb = *transb;
+  const int m = *pm, n = *pn, k = *pk, lda = *plda, ldb = *pldb, ldc = *pldc;
+  const double alpha = *palpha, beta = *pbeta;
+  if (dgemm_info(ta, tb, m, n, k, lda, ldb, ldc)) return;  // XERBLA is silent (xerbla.f)
+  if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return;
+  oa_context *h = default_handle();
+  if (!h) {
+    fprintf(stderr, "[oa_ortho] dgemm_: no usable gfx950 device; C left unchanged\n");
+    return;
+  }
+  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+  const bool need_ab = (alpha != 0.0 && k != 0);
+  const size_t na = need_ab ? span_elems(lda, nota ? m : k, nota ? k : m) : 0;
+  const size_t nb = need_ab ? span_elems(ldb, notb ? k : n, notb ? n : k) : 0;
+  const size_t nc = span_elems(ldc, m, n);
+  const bool dev = (!need_ab || (is_device_ptr(a) && is_device_ptr(b))) && is_device_ptr(c);
+  hipStream_t s = h->stream;
+  int st;
+  if (dev) {
+    st = enqueue_dgemm(s, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+    if (st == OA_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = OA_ERR_HIP;
+  } else {
+    // Host buffers (Charm++ message payloads): stage through the workspace.
+    const size_t al = 32;
+    auto up = [&](size_t e) { return (e + al - 1) / al * al; };
+    st = oa_ws_require(h, (up(na) + up(nb) + up(nc)) * sizeof(double));
+    if (st == OA_SUCCESS) {
+      double *da = (double *)h->ws.ptr;
+      double *db = da + up(na);
+      double *dc = db + up(nb);
+      hipError_t e = hipSuccess;
+      if (na) e = hipMemcpyAsync(da, a, na * sizeof(double), hipMemcpyHostToDevice, s);
+      if (e == hipSuccess && nb)
+        e = hipMemcpyAsync(db, b, nb * sizeof(double), hipMemcpyHostToDevice, s);
+      if (e == hipSuccess && beta != 0.0)
+        e = hipMemcpyAsync(dc, c, nc * sizeof(double), hipMemcpyHostToDevice, s);
+      if (e == hipSuccess && beta == 0.0 && ldc != m) {
+        // rows between m and ldc are not part of C: carry them through unchanged
+        e = hipMemcpyAsync(dc, c, nc * sizeof(double), hipMemcpyHostToDevice, s);
+      }
+      if (e == hipSuccess)
+        st = enqueue_dgemm(s, ta, tb, m, n, k, alpha, da, lda, db, ldb, beta, dc, ldc);
+      else
+        st = OA_ERR_HIP;
+      if (st == OA_SUCCESS &&
+          hipMemcpyAsync(c, dc, nc * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+        st = OA_ERR_HIP;
+      if (hipStreamSynchronize(s) != hipSuccess) st = OA_ERR_HIP;
+    }
+  }
+  if (st != OA_SUCCESS)
+    fprintf(stderr, "[oa_ortho] dgemm_ failed: %s\n", oa_status_string(st));
+}
+
+// ------------------------------------------------------------- path level
+
+int oa_overlap_sym(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, int nkx0,
+                   double alpha, int prec, double *S, double diag, double *maxdev) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (n > 0 && npw > 0 && !psi) return 4;
+  if (ldpsi < 2L * npw) return 5;
+  if (nkx0 < 0 || nkx0 > npw) return 6;
+  if (n > 0 && !S) return 9;
+  if (n == 0) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  const int K = 2 * npw;
+  if ((prec & 0xff) == OA_PREC_FP64) {
+    int st = oa_ws_require(c, oa_overlap_f64_ws_bytes(n, K));
+    if (st) return st;
+    return oa_launch_overlap_f64(c->stream, n, K, psi, ldpsi, 2 * nkx0, alpha, S, diag, maxdev,
+                                 c->ws.ptr, c->ws.bytes);
+  }
+  int st = oa_ws_require(c, oa_split_ws_bytes(prec, n, K));
+  if (st) return st;
+  return oa_launch_overlap_split(c->stream, prec, n, K, psi, ldpsi, 2 * nkx0, alpha, S, diag,
+                                 maxdev, c->ws.ptr, c->ws.bytes);
+}
+
+int oa_lambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, const double *F,
+              long ldF, double alpha, int prec, double *L) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (ldpsi < 2L * npw) return 5;
+  if (ldF < 2L * npw) return 7;
+  if (n == 0) return OA_SUCCESS;
+  if ((prec & 0xff) != OA_PREC_FP64) return OA_ERR_UNSUPPORTED;
+  DeviceGuard g(c->device);
+  const int K = 2 * npw;
+  int st = oa_ws_require(c, oa_ntsplit_f64_ws_bytes(n, n, K));
+  if (st) return st;
+  return oa_launch_ntsplit_f64(c->stream, n, n, K, psi, ldpsi, F, ldF, alpha, L, n, c->ws.ptr,
+                               c->ws.bytes);
+}
+
+static int s_to_t_sync(oa_context *c, int n, const double *S, int nspin, double tol, int maxit,
+                       double *T, int *iters, double *err, void *ws) {
+  int st = oa_ns_begin(c->stream, n, S, nspin, c->report, ws);
+  if (st) return st;
+  // Host loop with early exit: two guarded iterations per report read.
+  int it = 0;
+  for (;;) {
+    const int upto = (it + 2 < maxit) ? it + 2 : maxit;
+    for (; it < upto; it++) {
+      st = oa_ns_iterate(c->stream, n, it, tol, maxit, c->report, ws);
+      if (st) return st;
+    }
+    OA_HIP_CHECK(hipMemcpyAsync(c->host_report, c->report, sizeof(oa_ns_report),
+                                hipMemcpyDeviceToHost, c->stream));
+    OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->host_report->done || it >= maxit) break;
+  }
+  st = oa_ns_finish(c->stream, n, T, c->report, ws);
+  if (st) return st;
+  OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (iters) *iters = c->host_report->iterations;
+  if (err) *err = c->host_report->error;
+  return c->host_report->done ? c->host_report->status : OA_ERR_NOT_CONVERGED;
+}
+
+int oa_s_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int maxit,
+              double *T, int *iters, double *err) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (n > 0 && !S) return 3;
+  if (nspin != 1 && nspin != 2) return 4;
+  if (maxit < 1) return 6;
+  if (n > 0 && !T) return 7;
+  if (n == 0) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  int st = oa_ws_require(c, oa_ns_ws_bytes(n));
+  if (st) return st;
+  return s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ws.ptr);
+}
+
+int oa_s_to_t_async(oa_handle_t c, int n, const double *S, int nspin, double tol, int maxit,
+                    double *T, oa_ns_report *report) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (n > 0 && !S) return 3;
+  if (nspin != 1 && nspin != 2) return 4;
+  if (maxit < 1) return 6;
+  if (n > 0 && !T) return 7;
+  if (!report) return 8;
+  if (n == 0) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  int st = oa_ws_require(c, oa_ns_ws_bytes(n));
+  if (st) return st;
+  return oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, report, c->ws.ptr);
+}
+
+int oa_backmul(oa_handle_t c, int n, int npw, const double *M, int transM, const double *psi,
+               long ldpsi, double alpha, double beta, double *out, long ldout, int prec) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (n > 0 && !M) return 4;
+  if (ldpsi < 2L * npw) return 7;
+  if (ldout < 2L * npw) return 11;
+  if (n == 0 || npw == 0) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  const int K = 2 * npw;
+  if ((prec & 0xff) != OA_PREC_FP64) {
+    int st = oa_ws_require(c, oa_split_ws_bytes(prec, n, K));
+    if (st) return st;
+    return oa_launch_backmul_split(c->stream, prec, n, K, M, transM, psi, ldpsi, alpha, beta,
+                                   out, ldout, c->ws.ptr, c->ws.bytes);
+  }
+  oa_gemm_problem p;
+  p.M = n;
+  p.N = K;
+  p.K = n;
+  p.X = M;
+  p.ldx = n;
+  p.Y = psi;
+  p.ldy = ldpsi;
+  p.C = out;
+  p.ldc = ldout;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.diag_add = 0.0;
+  return oa_launch_gemm_f64(c->stream, transM != 0, false, &p, 1, nullptr);
+}
+
+int oa_ortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, int nkx0,
+                  int nspin, double tol, int maxit, int prec, double *S, double *T,
+                  double *psi_out, long ldout, int *iters, double *err) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (n > 0 && !psi) return 4;
+  if (ldpsi < 2L * npw) return 5;
+  if (nkx0 < 0 || nkx0 > npw) return 6;
+  if (nspin != 1 && nspin != 2) return 7;
+  if (maxit < 1) return 9;
+  if (n > 0 && (!S || !T || !psi_out)) return 11;
+  if (ldout < 2L * npw) return 14;
+  if (n == 0) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  int st = oa_ws_require(c, ws_for(n, npw) > oa_split_ws_bytes(prec, n, 2 * npw)
+                                ? ws_for(n, npw)
+                                : oa_split_ws_bytes(prec, n, 2 * npw));
+  if (st) return st;
+  st = oa_overlap_sym(c, n, npw, psi, ldpsi, nkx0, 2.0, prec, S, 0.0, nullptr);
+  if (st) return st;
+  if (iters || err) {
+    const int st_ns = s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ws.ptr);
+    if (st_ns != OA_SUCCESS && st_ns != OA_ERR_NOT_CONVERGED) return st_ns;
+    st = oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, prec);
+    return st ? st : st_ns;
+  }
+  st = oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, c->report, c->ws.ptr);
+  if (st) return st;
+  return oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, prec);
+}
+
+}  // extern "C"

@@ -1,0 +1,546 @@
+// FP64-MFMA kernels of the orthonormalization path (gfx950).
+//
+//   gemm_f64_kernel     row-major C = alpha*op(X)*op(Y) + beta*C + d*I, the
+//                       back-multiply T*psi / Lambda*psi and the S->T GEMMs.
+//   ovl_f64_kernel      split-K product over the plane-wave index, upper
+//                       triangle tiles only for the symmetric overlap (and
+//                       all tiles for the asymmetric Lambda product).
+//   ovl_reduce_kernel   fixed-order sum of the split-K partials with the kx=0
+//                       weight, mirror to the lower triangle, max|S - d I|.
+//   ns_* kernels        guarded Newton-Schulz S->T iteration (device flags).
+//
+// Tiling: 256-thread workgroups (4 wave64s in a 2x2 grid), each wave owns a
+// (16*WM) x (16*WM) block of v_mfma_f64_16x16x4f64 accumulators.  Operand
+// panels of BK=16 doubles are staged through LDS with a row stride of 18
+// doubles, which makes the 16-byte fragment reads of 16 lanes cover all 64
+// banks once.  Each lane reads two consecutive k per LDS access and feeds
+// them to two MFMAs; the k order inside a panel is therefore permuted the
+// same way for both operands, which leaves every product sum unchanged.
+#include "oa_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+constexpr int NT = 256;
+constexpr int BK = 16;
+constexpr int LS = BK + 2;
+
+__host__ __device__ __forceinline__ int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// The dispatcher places workgroup b on XCD b % 8.  Give each XCD a contiguous
+// range of logical ids so workgroups sharing operand panels share one L2.
+__device__ __forceinline__ int xcd_logical(int b, int nb) {
+  const int per = nb >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// BR x BK operand panel -> registers.  KC: element (r, k) is p[r*ld + k]
+// (k contiguous); otherwise p[k*ld + r] (r contiguous).  Out-of-range
+// elements read as zero.
+template <int BR, bool KC>
+__device__ __forceinline__ void panel_load(const double *__restrict__ p, long ld,
+                                           int r0, int rlim, int k0, int klim,
+                                           double (&v)[BR * BK / NT], int tid) {
+  constexpr int CNT = BR * BK / NT;
+  if (KC) {
+    const int kk = tid % BK, rr = tid / BK;
+    const int k = k0 + kk;
+#pragma unroll
+    for (int i = 0; i < CNT; i++) {
+      const int r = r0 + rr + i * (NT / BK);
+      v[i] = (r < rlim && k < klim) ? p[(long)r * ld + k] : 0.0;
+    }
+  } else {
+    const int rr = tid % BR, kk = tid / BR;
+    const int r = r0 + rr;
+#pragma unroll
+    for (int i = 0; i < CNT; i++) {
+      const int k = k0 + kk + i * (NT / BR);
+      v[i] = (r < rlim && k < klim) ? p[(long)k * ld + r] : 0.0;
+    }
+  }
+}
+
+template <int BR, bool KC>
+__device__ __forceinline__ void panel_store(double *s, const double (&v)[BR * BK / NT],
+                                            int tid) {
+  constexpr int CNT = BR * BK / NT;
+  if (KC) {
+    const int kk = tid % BK, rr = tid / BK;
+#pragma unroll
+    for (int i = 0; i < CNT; i++) s[(rr + i * (NT / BK)) * LS + kk] = v[i];
+  } else {
+    const int rr = tid % BR, kk = tid / BR;
+#pragma unroll
+    for (int i = 0; i < CNT; i++) s[rr * LS + kk + i * (NT / BR)] = v[i];
+  }
+}
+
+// One BK-deep LDS stage of MFMAs for a wave's WM x WM block grid.
+template <int WM>
+__device__ __forceinline__ void mma_stage(const double *Xs, const double *Ys, int xrow0,
+                                          int yrow0, int lane, d4 (&acc)[WM][WM]) {
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int kk2 = 0; kk2 < BK / 8; kk2++) {
+    d2v a[WM], b[WM];
+#pragma unroll
+    for (int i = 0; i < WM; i++)
+      a[i] = *reinterpret_cast<const d2v *>(Xs + (xrow0 + i * 16 + r) * LS + kk2 * 8 + 2 * g);
+#pragma unroll
+    for (int j = 0; j < WM; j++)
+      b[j] = *reinterpret_cast<const d2v *>(Ys + (yrow0 + j * 16 + r) * LS + kk2 * 8 + 2 * g);
+#pragma unroll
+    for (int e = 0; e < 2; e++)
+#pragma unroll
+      for (int i = 0; i < WM; i++)
+#pragma unroll
+        for (int j = 0; j < WM; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][e], b[j][e], acc[i][j], 0, 0, 0);
+  }
+}
+
+// Main K loop shared by every FP64 kernel: register-prefetched panels, one
+// LDS buffer per operand, two barriers per stage.  When YSAME the Y panel is
+// the X panel (diagonal tile of a symmetric product) and is loaded once.
+template <int WM, bool XKC, bool YKC, bool YSAME>
+__device__ __forceinline__ void kloop(const double *X, long ldx, int m0, int mlim,
+                                      const double *Y, long ldy, int n0, int nlim,
+                                      int ka, int kb, double *Xs, double *Ys,
+                                      d4 (&acc)[WM][WM]) {
+  constexpr int BM = 32 * WM;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
+  double rx[BM * BK / NT], ry[BM * BK / NT];
+  panel_load<BM, XKC>(X, ldx, m0, mlim, ka, kb, rx, tid);
+  if (!YSAME) panel_load<BM, YKC>(Y, ldy, n0, nlim, ka, kb, ry, tid);
+  for (int k0 = ka; k0 < kb; k0 += BK) {
+    __syncthreads();
+    panel_store<BM, XKC>(Xs, rx, tid);
+    if (!YSAME) panel_store<BM, YKC>(Ys, ry, tid);
+    __syncthreads();
+    if (k0 + BK < kb) {
+      panel_load<BM, XKC>(X, ldx, m0, mlim, k0 + BK, kb, rx, tid);
+      if (!YSAME) panel_load<BM, YKC>(Y, ldy, n0, nlim, k0 + BK, kb, ry, tid);
+    }
+    mma_stage<WM>(Xs, YSAME ? Xs : Ys, xrow0, yrow0, lane, acc);
+  }
+}
+
+struct GemmBatch {
+  oa_gemm_problem p[2];
+};
+
+template <bool TX, bool TY, int WM>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch gb, const int32_t *skip) {
+  if (skip && *skip) return;
+  const oa_gemm_problem &P = gb.p[blockIdx.y];
+  constexpr int BM = 32 * WM;
+  __shared__ __attribute__((aligned(16))) double Xs[BM * LS];
+  __shared__ __attribute__((aligned(16))) double Ys[BM * LS];
+  const int tmn = cdiv(P.M, BM), tnn = cdiv(P.N, BM);
+  const int logical = xcd_logical(blockIdx.x, gridDim.x);
+  if (logical >= tmn * tnn) return;
+  const int tm = logical % tmn, tn = logical / tmn;
+  const int m0 = tm * BM, n0 = tn * BM;
+  d4 acc[WM][WM];
+#pragma unroll
+  for (int i = 0; i < WM; i++)
+#pragma unroll
+    for (int j = 0; j < WM; j++) acc[i][j] = d4{0, 0, 0, 0};
+  kloop<WM, !TX, TY, false>(P.X, P.ldx, m0, P.M, P.Y, P.ldy, n0, P.N, 0, P.K, Xs, Ys, acc);
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < WM; i++)
+#pragma unroll
+    for (int j = 0; j < WM; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = m0 + xrow0 + i * 16 + 4 * g + v;
+        const int n = n0 + yrow0 + j * 16 + r;
+        if (m < P.M && n < P.N) {
+          double *c = P.C + (long)m * P.ldc + n;
+          double val = P.alpha * acc[i][j][v];
+          if (P.beta != 0.0) val += P.beta * (*c);
+          if (m == n && P.diag_add != 0.0) val += P.diag_add;
+          *c = val;
+        }
+      }
+}
+
+// Split-K partial products over the plane-wave index.
+//   SYM: tiles (bi <= bj) of X X^T; else all tiles of X Y^T.
+//   Chunk 0 covers [0, kx0_end) when kx0_end > 0 (weighted 1/2 later).
+struct OvlArgs {
+  const double *X;
+  long ldx;
+  const double *Y;
+  long ldy;
+  int M, N, K;
+  int kx0_end;
+  int nz_main, kchunk;
+  double *part;  // [nz][ntiles][BM][BM]
+};
+
+__device__ __forceinline__ void upper_tile(int tile, int T, int &bi, int &bj) {
+  int b = 0, rem = tile;
+  while (rem >= T - b) {
+    rem -= T - b;
+    b++;
+  }
+  bi = b;
+  bj = b + rem;
+}
+
+template <int WM, bool SYM>
+__global__ __launch_bounds__(NT) void ovl_f64_kernel(OvlArgs a) {
+  constexpr int BM = 32 * WM;
+  __shared__ __attribute__((aligned(16))) double Xs[BM * LS];
+  __shared__ __attribute__((aligned(16))) double Ys[BM * LS];
+  const int Tm = cdiv(a.M, BM), Tn = cdiv(a.N, BM);
+  const int ntiles = SYM ? Tm * (Tm + 1) / 2 : Tm * Tn;
+  const int nz = a.nz_main + (a.kx0_end > 0 ? 1 : 0);
+  const int logical = xcd_logical(blockIdx.x, gridDim.x);
+  if (logical >= ntiles * nz) return;
+  const int tile = logical % ntiles, z = logical / ntiles;
+  int bi, bj;
+  if (SYM) upper_tile(tile, Tm, bi, bj);
+  else { bi = tile % Tm; bj = tile / Tm; }
+  int ka, kb;
+  if (a.kx0_end > 0) {
+    if (z == 0) { ka = 0; kb = a.kx0_end; }
+    else { ka = a.kx0_end + (z - 1) * a.kchunk; kb = min(a.K, ka + a.kchunk); }
+  } else {
+    ka = z * a.kchunk;
+    kb = min(a.K, ka + a.kchunk);
+  }
+  d4 acc[WM][WM];
+#pragma unroll
+  for (int i = 0; i < WM; i++)
+#pragma unroll
+    for (int j = 0; j < WM; j++) acc[i][j] = d4{0, 0, 0, 0};
+  const int m0 = bi * BM, n0 = bj * BM;
+  if (ka < kb) {
+    if (SYM && bi == bj)
+      kloop<WM, true, true, true>(a.X, a.ldx, m0, a.M, a.X, a.ldx, n0, a.N, ka, kb, Xs, Ys, acc);
+    else
+      kloop<WM, true, true, false>(a.X, a.ldx, m0, a.M, SYM ? a.X : a.Y, SYM ? a.ldx : a.ldy,
+                                   n0, a.N, ka, kb, Xs, Ys, acc);
+  }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
+  const int r = lane & 15, g = lane >> 4;
+  double *pt = a.part + ((long)z * ntiles + tile) * (BM * BM);
+#pragma unroll
+  for (int i = 0; i < WM; i++)
+#pragma unroll
+    for (int j = 0; j < WM; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int lm = xrow0 + i * 16 + 4 * g + v;
+        const int ln = yrow0 + j * 16 + r;
+        pt[lm * BM + ln] = acc[i][j][v];
+      }
+}
+
+template <int WM, bool SYM>
+__global__ __launch_bounds__(NT) void ovl_reduce_kernel(const double *__restrict__ part, int M,
+                                                       int N, int nz, int has_kx0,
+                                                       double alpha, double *C, long ldc,
+                                                       double diag,
+                                                       unsigned long long *maxdev) {
+  constexpr int BM = 32 * WM;
+  const int Tm = cdiv(M, BM), Tn = cdiv(N, BM);
+  const int ntiles = SYM ? Tm * (Tm + 1) / 2 : Tm * Tn;
+  const long idx = (long)blockIdx.x * NT + threadIdx.x;
+  const int tile = (int)(idx / (BM * BM));
+  if (tile >= ntiles) return;
+  const int e = (int)(idx % (BM * BM));
+  const int lm = e / BM, ln = e % BM;
+  int bi, bj;
+  if (SYM) upper_tile(tile, Tm, bi, bj);
+  else { bi = tile % Tm; bj = tile / Tm; }
+  if (SYM && bi == bj && ln < lm) return;
+  const int i = bi * BM + lm, j = bj * BM + ln;
+  if (i >= M || j >= N) return;
+  double s = 0.0;
+  for (int z = 0; z < nz; z++) {
+    const double p = part[((long)z * ntiles + tile) * (BM * BM) + e];
+    s += (has_kx0 && z == 0) ? 0.5 * p : p;
+  }
+  s *= alpha;
+  C[(long)i * ldc + j] = s;
+  if (SYM) C[(long)j * ldc + i] = s;
+  if (maxdev) {
+    const double dv = fabs(s - (i == j ? diag : 0.0));
+    atomicMax(maxdev, (unsigned long long)__double_as_longlong(dv));
+  }
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, unsigned long long v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ S->T
+// Ping-pong buffers A[2], B[2] and C; iteration i reads index i%2 and writes
+// 1-i%2, so the final T sits in A[iterations % 2].
+struct NsBufs {
+  double *A[2];
+  double *B[2];
+  double *C;
+  double *partial;  // NS_RES_WG doubles
+};
+constexpr int NS_RES_WG = 128;
+
+__global__ void ns_init_kernel(const double *__restrict__ S, int n, int nspin, double *A,
+                               double *B, oa_ns_report *rep) {
+  const long nn = (long)n * n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < nn;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    B[idx] = (nspin == 1) ? S[idx] / 2.0 : S[idx];
+    A[idx] = (i == j) ? 1.0 : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rep->iterations = 0;
+    rep->done = 0;
+    rep->status = OA_SUCCESS;
+    rep->pad = 0;
+    rep->error = 0.0;
+  }
+}
+
+// Fixed-order partial sums of (Aold - Anew)^2 (ortho.C:949-953).
+__global__ __launch_bounds__(NT) void ns_resid_kernel(const double *__restrict__ Aold,
+                                                      const double *__restrict__ Anew, long nn,
+                                                      double *partial, const int32_t *skip) {
+  if (*skip) return;
+  __shared__ double red[NT];
+  double s = 0.0;
+  for (long idx = (long)blockIdx.x * NT + threadIdx.x; idx < nn; idx += (long)gridDim.x * NT) {
+    const double d = Aold[idx] - Anew[idx];
+    s += d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = NT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Ortho::collect_error (ortho.C:109-146) on the device.
+__global__ void ns_update_kernel(const double *__restrict__ partial, int nwg, int n,
+                                 double tol, int maxit, oa_ns_report *rep) {
+  if (threadIdx.x != 0 || rep->done) return;
+  double s = 0.0;
+  for (int i = 0; i < nwg; i++) s += partial[i];
+  const double err = sqrt(s / ((double)n * (double)n));
+  const int it = rep->iterations + 1;
+  rep->iterations = it;
+  rep->error = err;
+  if (!(err > tol && it < maxit)) {
+    rep->done = 1;
+    rep->status = (it >= maxit) ? OA_ERR_NOT_CONVERGED : OA_SUCCESS;
+  }
+}
+
+__global__ void ns_finish_kernel(const double *__restrict__ A0, const double *__restrict__ A1,
+                                 long nn, double *T, const oa_ns_report *rep) {
+  const double *src = (rep->iterations & 1) ? A1 : A0;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < nn;
+       idx += (long)gridDim.x * blockDim.x)
+    T[idx] = src[idx];
+}
+
+template <bool TX, bool TY>
+int launch_gemm(hipStream_t s, const oa_gemm_problem *p, int nprob, const int32_t *skip) {
+  int tiles128 = 0;
+  for (int i = 0; i < nprob; i++)
+    tiles128 = max(tiles128, cdiv(p[i].M, 128) * cdiv(p[i].N, 128));
+  GemmBatch gb;
+  for (int i = 0; i < nprob; i++) gb.p[i] = p[i];
+  if (tiles128 * nprob >= 512) {
+    const int nb = cdiv(tiles128, 8) * 8;
+    hipLaunchKernelGGL((gemm_f64_kernel<TX, TY, 4>), dim3(nb, nprob), dim3(NT), 0, s, gb, skip);
+  } else {
+    int tiles64 = 0;
+    for (int i = 0; i < nprob; i++)
+      tiles64 = max(tiles64, cdiv(p[i].M, 64) * cdiv(p[i].N, 64));
+    const int nb = cdiv(tiles64, 8) * 8;
+    hipLaunchKernelGGL((gemm_f64_kernel<TX, TY, 2>), dim3(nb, nprob), dim3(NT), 0, s, gb, skip);
+  }
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+// Split-K plan for the plane-wave products.
+struct OvlPlan {
+  int WM, BM, ntiles, nz_main, kchunk, nz;
+};
+
+OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym) {
+  OvlPlan pl;
+  const int t128 = sym ? cdiv(M, 128) * (cdiv(M, 128) + 1) / 2 : cdiv(M, 128) * cdiv(N, 128);
+  pl.WM = (M >= 256 || t128 >= 16) ? 4 : 2;
+  pl.BM = 32 * pl.WM;
+  const int Tm = cdiv(M, pl.BM), Tn = cdiv(N, pl.BM);
+  pl.ntiles = sym ? Tm * (Tm + 1) / 2 : Tm * Tn;
+  const int Kmain = K - kx0_end;
+  // aim for ~1024 workgroups, chunks of at least 8 stages
+  int nz = cdiv(1024, pl.ntiles);
+  nz = min(nz, max(1, cdiv(Kmain, 8 * BK)));
+  nz = max(1, min(nz, 512));
+  int kchunk = cdiv(cdiv(max(Kmain, 1), nz), BK) * BK;
+  nz = max(1, cdiv(max(Kmain, 1), kchunk));
+  pl.nz_main = nz;
+  pl.kchunk = kchunk;
+  pl.nz = nz + (kx0_end > 0 ? 1 : 0);
+  return pl;
+}
+
+}  // namespace
+
+int oa_launch_gemm_f64(hipStream_t s, bool tx, bool ty, const oa_gemm_problem *p, int nprob,
+                       const int32_t *skip) {
+  if (nprob < 1 || nprob > 2) return OA_ERR_UNSUPPORTED;
+  if (!tx && !ty) return launch_gemm<false, false>(s, p, nprob, skip);
+  if (!tx && ty) return launch_gemm<false, true>(s, p, nprob, skip);
+  if (tx && !ty) return launch_gemm<true, false>(s, p, nprob, skip);
+  return launch_gemm<true, true>(s, p, nprob, skip);
+}
+
+size_t oa_overlap_f64_ws_bytes(int n, int K) {
+  const OvlPlan pl = plan_ovl(n, n, K, 0, true);
+  // kx0 chunk adds one more z slice at most; maxdev slot after the partials
+  return (size_t)(pl.nz + 1) * pl.ntiles * pl.BM * pl.BM * sizeof(double) + 256;
+}
+
+size_t oa_ntsplit_f64_ws_bytes(int M, int N, int K) {
+  const OvlPlan pl = plan_ovl(M, N, K, 0, false);
+  return (size_t)pl.nz * pl.ntiles * pl.BM * pl.BM * sizeof(double) + 256;
+}
+
+template <int WM, bool SYM>
+static int launch_ovl(hipStream_t s, const OvlPlan &pl, OvlArgs a, int M, int N, int kx0_end,
+                      double alpha, double *C, long ldc, double diag,
+                      unsigned long long *maxdev) {
+  const int G = pl.ntiles * pl.nz;
+  const int nb = cdiv(G, 8) * 8;
+  hipLaunchKernelGGL((ovl_f64_kernel<WM, SYM>), dim3(nb), dim3(NT), 0, s, a);
+  if (maxdev) hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, maxdev, 0ull, 1);
+  const long total = (long)pl.ntiles * pl.BM * pl.BM;
+  hipLaunchKernelGGL((ovl_reduce_kernel<WM, SYM>), dim3((unsigned)cdiv((int)total, NT)),
+                     dim3(NT), 0, s, (const double *)a.part, M, N, pl.nz,
+                     kx0_end > 0 ? 1 : 0, alpha, C, ldc, diag, maxdev);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+int oa_launch_overlap_f64(hipStream_t s, int n, int K, const double *psi, long ld, int kx0_end,
+                          double alpha, double *S, double diag, double *maxdev, void *ws,
+                          size_t ws_bytes) {
+  const OvlPlan pl = plan_ovl(n, n, K, kx0_end, true);
+  const size_t need = (size_t)pl.nz * pl.ntiles * pl.BM * pl.BM * sizeof(double);
+  if (need > ws_bytes) return OA_ERR_CAPACITY;
+  OvlArgs a;
+  a.X = psi; a.ldx = ld; a.Y = psi; a.ldy = ld;
+  a.M = n; a.N = n; a.K = K;
+  a.kx0_end = kx0_end; a.nz_main = pl.nz_main; a.kchunk = pl.kchunk;
+  a.part = (double *)ws;
+  unsigned long long *md = reinterpret_cast<unsigned long long *>(maxdev);
+  if (pl.WM == 4) return launch_ovl<4, true>(s, pl, a, n, n, kx0_end, alpha, S, n, diag, md);
+  return launch_ovl<2, true>(s, pl, a, n, n, kx0_end, alpha, S, n, diag, md);
+}
+
+int oa_launch_ntsplit_f64(hipStream_t s, int M, int N, int K, const double *X, long ldx,
+                          const double *Y, long ldy, double alpha, double *C, long ldc, void *ws,
+                          size_t ws_bytes) {
+  const OvlPlan pl = plan_ovl(M, N, K, 0, false);
+  const size_t need = (size_t)pl.nz * pl.ntiles * pl.BM * pl.BM * sizeof(double);
+  if (need > ws_bytes) return OA_ERR_CAPACITY;
+  OvlArgs a;
+  a.X = X; a.ldx = ldx; a.Y = Y; a.ldy = ldy;
+  a.M = M; a.N = N; a.K = K;
+  a.kx0_end = 0; a.nz_main = pl.nz_main; a.kchunk = pl.kchunk;
+  a.part = (double *)ws;
+  if (pl.WM == 4) return launch_ovl<4, false>(s, pl, a, M, N, 0, alpha, C, ldc, 0.0, nullptr);
+  return launch_ovl<2, false>(s, pl, a, M, N, 0, alpha, C, ldc, 0.0, nullptr);
+}
+
+size_t oa_ns_ws_bytes(int n) {
+  return (size_t)5 * n * n * sizeof(double) + NS_RES_WG * sizeof(double) + 256;
+}
+
+// S->T on the device in three parts so a host loop can stop early:
+// begin (B = S/2 or S, A = I, report cleared), guarded iterations, finish.
+static void ns_bufs(void *ws, int n, NsBufs &b) {
+  const long nn = (long)n * n;
+  double *base = (double *)ws;
+  b.A[0] = base;
+  b.A[1] = base + nn;
+  b.B[0] = base + 2 * nn;
+  b.B[1] = base + 3 * nn;
+  b.C = base + 4 * nn;
+  b.partial = base + 5 * nn;
+}
+
+int oa_ns_begin(hipStream_t s, int n, const double *S, int nspin, oa_ns_report *rep, void *ws) {
+  NsBufs b;
+  ns_bufs(ws, n, b);
+  const long nn = (long)n * n;
+  const long nblk = (nn + NT - 1) / NT;
+  const int eb = (int)(nblk < 1 ? 1 : (nblk > 1024 ? 1024 : nblk));
+  hipLaunchKernelGGL(ns_init_kernel, dim3(eb), dim3(NT), 0, s, S, n, nspin, b.A[0], b.B[0], rep);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+// Iteration `it` (0-based) of Ortho::do_iteration .. collect_error; a no-op
+// once rep->done is set.
+int oa_ns_iterate(hipStream_t s, int n, int it, double tol, int maxit, oa_ns_report *rep,
+                  void *ws) {
+  NsBufs b;
+  ns_bufs(ws, n, b);
+  const long nn = (long)n * n;
+  const int32_t *skip = &rep->done;
+  const int cur = it & 1, nxt = cur ^ 1;
+  // do_iteration (ortho.C:848-870): C = 3I - A*B
+  oa_gemm_problem p1{n, n, n, b.A[cur], n, b.B[cur], n, b.C, n, -1.0, 0.0, 3.0};
+  int st = oa_launch_gemm_f64(s, false, false, &p1, 1, skip);
+  if (st) return st;
+  // step_2 / step_3 (ortho.C:880-929): B' = B*C/2 ; A' = C*A/2
+  oa_gemm_problem p2[2] = {{n, n, n, b.B[cur], n, b.C, n, b.B[nxt], n, 0.5, 0.0, 0.0},
+                           {n, n, n, b.C, n, b.A[cur], n, b.A[nxt], n, 0.5, 0.0, 0.0}};
+  st = oa_launch_gemm_f64(s, false, false, p2, 2, skip);
+  if (st) return st;
+  hipLaunchKernelGGL(ns_resid_kernel, dim3(NS_RES_WG), dim3(NT), 0, s, (const double *)b.A[cur],
+                     (const double *)b.A[nxt], nn, b.partial, skip);
+  hipLaunchKernelGGL(ns_update_kernel, dim3(1), dim3(64), 0, s, (const double *)b.partial,
+                     NS_RES_WG, n, tol, maxit, rep);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+int oa_ns_finish(hipStream_t s, int n, double *T, const oa_ns_report *rep, void *ws) {
+  NsBufs b;
+  ns_bufs(ws, n, b);
+  const long nn = (long)n * n;
+  const long nblk = (nn + NT - 1) / NT;
+  const int eb = (int)(nblk < 1 ? 1 : (nblk > 1024 ? 1024 : nblk));
+  hipLaunchKernelGGL(ns_finish_kernel, dim3(eb), dim3(NT), 0, s, (const double *)b.A[0],
+                     (const double *)b.A[1], nn, T, rep);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+int oa_launch_s_to_t(hipStream_t s, int n, const double *S, int nspin, double tol, int maxit,
+                     double *T, oa_ns_report *rep, void *ws) {
+  int st = oa_ns_begin(s, n, S, nspin, rep, ws);
+  for (int it = 0; it < maxit && st == OA_SUCCESS; it++)
+    st = oa_ns_iterate(s, n, it, tol, maxit, rep, ws);
+  if (st) return st;
+  return oa_ns_finish(s, n, T, rep, ws);
+}

@@ -1,0 +1,281 @@
+"""Host-side mirror of the reference's PairCalculator / Ortho interface for the
+orthonormalization path, over the C ABI of liboa_ortho.so.
+
+Device memory comes from PyTorch tensors (float64, CUDA/HIP device); the
+library runs every product on the GPU.  Names follow the reference:
+
+  PairCalculator.multiplyForward   ckPairCalculator.C:821-933 (symmetric
+                                   S with alpha=2 and the kx=0 weight, or the
+                                   asymmetric Lambda with alpha=1)
+  PairCalculator.multiplyResult    ckPairCalculator.C:1201-1468 ->
+                                   bwMultiplyHelper :1710-1946
+  PairCalculator.bwMultiplyDynOrthoT  ckPairCalculator.C:1965-2031 (-F*T)
+  PairCalculator.multiplyPsiV      ckPairCalculator.C:1128-1176
+  Ortho.start_calc                 ortho.C:153-266 + do_iteration..collect_error
+                                   (:848-971, :109-146)
+  Ortho.array_diag_max             ortho.h:282-313
+  Ortho.acceptSectionLambda        ortho.C:497-608 (lsda: Lambda x 2)
+  Ortho.gamma                      ortho.C:565-570 (Gamma = T * Lambda)
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import check, prec_code
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _need(t: torch.Tensor, name: str, rows: int | None = None) -> None:
+    if not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError(f"{name} must be a float64 device tensor")
+    if t.dim() != 2 or t.stride(1) != 1:
+        raise ValueError(f"{name} must be 2-D with unit column stride")
+    if rows is not None and t.shape[0] != rows:
+        raise ValueError(f"{name} must have {rows} rows")
+
+
+class Handle:
+    """An oa_handle_t bound to a device and (by default) to PyTorch's current
+    stream on that device, so library work orders with torch work."""
+
+    def __init__(self, device: int | None = None, stream: torch.cuda.Stream | None = None):
+        L = _lib.lib()
+        self.device = torch.cuda.current_device() if device is None else device
+        h = ctypes.c_void_p()
+        check(L.oa_create(ctypes.byref(h), self.device), "oa_create")
+        self.h = h
+        self.use_stream(stream)
+
+    def use_stream(self, stream: torch.cuda.Stream | None = None) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self.stream = s
+        check(_lib.lib().oa_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)), "oa_set_stream")
+
+    def reserve(self, n: int, npw: int) -> None:
+        check(_lib.lib().oa_reserve(self.h, n, npw), "oa_reserve")
+
+    def synchronize(self) -> None:
+        check(_lib.lib().oa_synchronize(self.h), "oa_synchronize")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _lib.lib().oa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- raw calls
+    def dgemm(self, transa: str, transb: str, m: int, n: int, k: int, alpha: float,
+              a: torch.Tensor, lda: int, b: torch.Tensor, ldb: int, beta: float,
+              c: torch.Tensor, ldc: int) -> int:
+        """Column-major DGEMM on device buffers (returns the BLAS INFO)."""
+        return _lib.lib().oa_dgemm(self.h, transa.encode(), transb.encode(), m, n, k, alpha,
+                                   _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
+
+    def overlap_sym(self, psi: torch.Tensor, nkx0: int = 1, alpha: float = 2.0,
+                    prec="fp64", S: torch.Tensor | None = None, diag: float = 0.0,
+                    maxdev: torch.Tensor | None = None) -> torch.Tensor:
+        n = psi.shape[0]
+        _need(psi, "psi")
+        npw = psi.shape[1] // 2
+        if S is None:
+            S = torch.empty((n, n), dtype=torch.float64, device=psi.device)
+        check(_lib.lib().oa_overlap_sym(self.h, n, npw, _ptr(psi), psi.stride(0), nkx0, alpha,
+                                        prec_code(prec), _ptr(S), diag, _ptr(maxdev)),
+              "oa_overlap_sym")
+        return S
+
+    def lambda_fwd(self, psi: torch.Tensor, F: torch.Tensor, alpha: float = 1.0,
+                   prec="fp64", L: torch.Tensor | None = None) -> torch.Tensor:
+        n = psi.shape[0]
+        _need(psi, "psi")
+        _need(F, "F", n)
+        npw = psi.shape[1] // 2
+        if L is None:
+            L = torch.empty((n, n), dtype=torch.float64, device=psi.device)
+        check(_lib.lib().oa_lambda(self.h, n, npw, _ptr(psi), psi.stride(0), _ptr(F),
+                                   F.stride(0), alpha, prec_code(prec), _ptr(L)), "oa_lambda")
+        return L
+
+    def s_to_t(self, S: torch.Tensor, nspin: int = 1, tol: float = 1e-15, maxit: int = 1000,
+               T: torch.Tensor | None = None):
+        """Synchronous S->T.  Returns (T, iterations, error); raises
+        OrthoNotConverged when the iteration limit was reached."""
+        n = S.shape[0]
+        _need(S, "S", n)
+        if T is None:
+            T = torch.empty_like(S)
+        it = ctypes.c_int(0)
+        err = ctypes.c_double(0)
+        st = _lib.lib().oa_s_to_t(self.h, n, _ptr(S), nspin, tol, maxit, _ptr(T),
+                                  ctypes.byref(it), ctypes.byref(err))
+        self.last_iterations, self.last_error = it.value, err.value
+        check(st, "oa_s_to_t")
+        return T, it.value, err.value
+
+    def s_to_t_async(self, S: torch.Tensor, T: torch.Tensor, report: torch.Tensor,
+                     nspin: int = 1, tol: float = 1e-15, maxit: int = 12) -> None:
+        """Enqueue `maxit` guarded iterations; `report` is a 24-byte device
+        tensor receiving an oa_ns_report."""
+        n = S.shape[0]
+        check(_lib.lib().oa_s_to_t_async(self.h, n, _ptr(S), nspin, tol, maxit, _ptr(T),
+                                         _ptr(report)), "oa_s_to_t_async")
+
+    def backmul(self, M: torch.Tensor, psi: torch.Tensor, transM: bool = False,
+                alpha: float = 1.0, beta: float = 0.0, out: torch.Tensor | None = None,
+                prec="fp64") -> torch.Tensor:
+        n = psi.shape[0]
+        _need(psi, "psi")
+        _need(M, "M", n)
+        npw = psi.shape[1] // 2
+        if out is None:
+            out = torch.zeros_like(psi) if beta != 0.0 else torch.empty_like(psi)
+        check(_lib.lib().oa_backmul(self.h, n, npw, _ptr(M), 1 if transM else 0, _ptr(psi),
+                                    psi.stride(0), alpha, beta, _ptr(out), out.stride(0),
+                                    prec_code(prec)), "oa_backmul")
+        return out
+
+    def ortho_step(self, psi: torch.Tensor, nkx0: int = 1, nspin: int = 1,
+                   tol: float = 1e-15, maxit: int = 1000, prec="fp64",
+                   S=None, T=None, out=None, sync: bool = True):
+        """One symmetric step.  Returns (S, T, psi_out, iterations, error);
+        iterations / error are None when sync is False."""
+        n = psi.shape[0]
+        _need(psi, "psi")
+        npw = psi.shape[1] // 2
+        dev = psi.device
+        S = torch.empty((n, n), dtype=torch.float64, device=dev) if S is None else S
+        T = torch.empty((n, n), dtype=torch.float64, device=dev) if T is None else T
+        out = torch.empty_like(psi) if out is None else out
+        it = ctypes.c_int(0)
+        err = ctypes.c_double(0)
+        st = _lib.lib().oa_ortho_step(self.h, n, npw, _ptr(psi), psi.stride(0), nkx0, nspin,
+                                      tol, maxit, prec_code(prec), _ptr(S), _ptr(T), _ptr(out),
+                                      out.stride(0), ctypes.byref(it) if sync else None,
+                                      ctypes.byref(err) if sync else None)
+        check(st, "oa_ortho_step")
+        if sync:
+            return S, T, out, it.value, err.value
+        return S, T, out, None, None
+
+
+_default: dict[int, Handle] = {}
+
+
+def default_handle(device: int | None = None) -> Handle:
+    d = torch.cuda.current_device() if device is None else device
+    h = _default.get(d)
+    if h is None:
+        h = _default[d] = Handle(d)
+    else:
+        h.use_stream()
+    return h
+
+
+class PairCalculator:
+    """One PairCalculator instance over a whole state block (the reference's
+    default decomposition sGrainSize = n collapses the 4-D array to one tile
+    per plane; here the plane split is the kernel's internal split-K).
+
+    symmetric=True  -> the psi x psi instance (S, T*psi); alpha = 2 when
+                       double packing is on (gamma point), kx=0 weighted.
+    symmetric=False -> the psi x F instance (Lambda, alpha = 1)."""
+
+    def __init__(self, symmetric: bool = True, nkx0: int = 1, double_pack: bool = True,
+                 prec="fp64", handle: Handle | None = None):
+        self.symmetric = symmetric
+        self.nkx0 = nkx0
+        self.double_pack = double_pack
+        self.prec = prec
+        self.handle = handle or default_handle()
+
+    def multiplyForward(self, left: torch.Tensor, right: torch.Tensor | None = None,
+                        diag: float | None = None):
+        """Symmetric: S (and, when `diag` is given, max|S - diag*I| for the
+        dynamics tolerance check, returned as (S, maxdev)).  Asymmetric:
+        Lambda from (psi, force)."""
+        h = self.handle
+        if self.symmetric:
+            alpha = 2.0 if self.double_pack else 1.0
+            md = None
+            if diag is not None:
+                md = torch.zeros(1, dtype=torch.float64, device=left.device)
+            S = h.overlap_sym(left, nkx0=self.nkx0 if self.double_pack else 0, alpha=alpha,
+                              prec=self.prec, diag=diag or 0.0, maxdev=md)
+            return S if md is None else (S, float(md.item()))
+        if right is None:
+            raise ValueError("asymmetric instance needs the right (force) data")
+        return h.lambda_fwd(left, right, alpha=1.0, prec=self.prec)
+
+    def multiplyResult(self, M: torch.Tensor, psi: torch.Tensor, out=None,
+                       alpha: float = 1.0, beta: float = 0.0):
+        """Symmetric: out = alpha*T*psi + beta*out (ckPairCalculator.C:1835);
+        asymmetric: out = alpha*Lambda^T*psi + beta*out (:1842)."""
+        return self.handle.backmul(M, psi, transM=not self.symmetric, alpha=alpha, beta=beta,
+                                   out=out, prec=self.prec)
+
+    def bwMultiplyDynOrthoT(self, orthoT: torch.Tensor, F: torch.Tensor, out=None,
+                            beta: float = 0.0):
+        """-F*T (ckPairCalculator.C:2018, alpha = -1)."""
+        return self.handle.backmul(orthoT, F, transM=False, alpha=-1.0, beta=beta, out=out,
+                                   prec=self.prec)
+
+    def multiplyPsiV(self, orthoT: torch.Tensor, psiV: torch.Tensor, out=None):
+        return self.handle.backmul(orthoT, psiV, transM=False, out=out, prec=self.prec)
+
+
+class Ortho:
+    """The Ortho chare array collapsed onto one GPU (orthoGrainSize = n)."""
+
+    def __init__(self, nspin: int = 1, invsqr_tolerance: float = 1e-15,
+                 invsqr_max_iter: int = 1000, max_tolerance: float | None = None,
+                 handle: Handle | None = None):
+        self.nspin = nspin
+        self.lsda = nspin != 1
+        self.invsqr_tolerance = invsqr_tolerance
+        self.invsqr_max_iter = invsqr_max_iter
+        self.max_tolerance = max_tolerance
+        self.handle = handle or default_handle()
+        self.iterations = 0
+        self.error = float("nan")
+
+    def start_calc(self, S: torch.Tensor) -> torch.Tensor:
+        T, self.iterations, self.error = self.handle.s_to_t(
+            S, nspin=self.nspin, tol=self.invsqr_tolerance, maxit=self.invsqr_max_iter)
+        return T
+
+    @property
+    def diag(self) -> float:
+        """The diagonal Ortho::array_diag_max measures against (ortho.h:
+        282-313): 1 under lsda, else 2.  Pass it to
+        PairCalculator.multiplyForward(diag=...) to get max|S - d I| from the
+        overlap epilogue."""
+        return 1.0 if self.lsda else 2.0
+
+    def tolerance_ok(self, maxdev: float) -> bool:
+        """Ortho::maxCheck (ortho.C:455-475): False means recompute PsiV."""
+        return self.max_tolerance is None or maxdev < self.max_tolerance
+
+    def acceptSectionLambda(self, L: torch.Tensor) -> torch.Tensor:
+        """Minimisation: Lambda (x2 under lsda) goes back to the asymmetric PCs."""
+        return L * 2.0 if self.lsda else L
+
+    def gamma(self, orthoT: torch.Tensor, L: torch.Tensor) -> torch.Tensor:
+        """Dynamics: Gamma = T * Lambda (CLA multiply alpha=1, beta=0)."""
+        n = L.shape[0]
+        Lx = L * 2.0 if self.lsda else L
+        G = torch.empty_like(L)
+        # row-major G = T * Lx  ==  column-major G^T = Lx^T * T^T
+        st = self.handle.dgemm("N", "N", n, n, n, 1.0, Lx, n, orthoT, n, 0.0, G, n)
+        check(st, "oa_dgemm")
+        return G

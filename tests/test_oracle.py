@@ -1,0 +1,258 @@
+"""Pins the CPU oracle before anything is checked against it.
+
+  * oao_dgemm is bitwise equal to the reference's compiled netlib dgemm_
+    (src_mathlib/dgemm.f) on all four transpose cases, alpha/beta special
+    cases and the XERBLA argument checks;
+  * the overlap matches a long-double brute force of the 2Re - G0 metric, and
+    its G=0 term reduces to the full-sphere complex overlap of a
+    Hermitian-symmetric state set;
+  * S->T matches the closed form V diag((lambda/2)^-1/2) V^T (what
+    get_diag_Tmat computes, cp_lowdin.C:136-153), its invariants and special
+    cases, PINY's serial iteration, and the reference's stop rule;
+  * the whole oracle step run on oao_dgemm and on the reference dgemm_ is
+    bitwise identical, and reproduces the committed golden fixtures.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from openatom_amd.workload import make_states, metric_weights
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def need_ref(o):
+    if not o.ref_available():
+        pytest.skip("reference BLAS not built (oracle/_ref)")
+
+
+# ------------------------------------------------------------------ DGEMM
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (2.0, 0.0), (-1.0, 1.0), (0.5, -0.25)])
+def test_dgemm_bitwise_vs_reference(oracle, ta, tb, alpha, beta):
+    need_ref(oracle)
+    rng = np.random.default_rng(5)
+    m, n, k = 23, 19, 31
+    lda = (m if ta == "N" else k) + 3
+    ldb = (k if tb == "N" else n) + 2
+    ldc = m + 1
+    A = rng.standard_normal(lda * (k if ta == "N" else m))
+    B = rng.standard_normal(ldb * (n if tb == "N" else k))
+    C0 = rng.standard_normal(ldc * n)
+    C1, C2 = C0.copy(), C0.copy()
+    oracle.dgemm_cm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C1, ldc, which="oracle")
+    oracle.dgemm_cm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C2, ldc, which="ref")
+    assert np.array_equal(C1, C2)
+    # and both equal the mathematical product to rounding
+    Am = A.reshape(-1, lda).T[:, :] if True else None
+    opA = (A.reshape(-1, lda).T[: (m if ta == "N" else k)])
+    opA = opA if ta == "N" else opA.T
+    opB = (B.reshape(-1, ldb).T[: (k if tb == "N" else n)])
+    opB = opB if tb == "N" else opB.T
+    Cm = C0.reshape(n, ldc).T[:m]
+    ref = alpha * (opA @ opB) + beta * Cm
+    got = C1.reshape(n, ldc).T[:m]
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+    # rows m..ldc-1 are not part of C and stay untouched
+    assert np.array_equal(C1.reshape(n, ldc)[:, m:], C0.reshape(n, ldc)[:, m:])
+
+
+def test_dgemm_argument_checks(oracle):
+    need_ref(oracle)
+    A = np.ones(16)
+    B = np.ones(16)
+    cases = [  # (ta, tb, m, n, k, lda, ldb, ldc) -> INFO (dgemm.f:169-190)
+        (("X", "N", 2, 2, 2, 2, 2, 2), 1), (("N", "Q", 2, 2, 2, 2, 2, 2), 2),
+        (("N", "N", -1, 2, 2, 2, 2, 2), 3), (("N", "N", 2, -1, 2, 2, 2, 2), 4),
+        (("N", "N", 2, 2, -1, 2, 2, 2), 5), (("N", "N", 4, 2, 2, 3, 2, 4), 8),
+        (("N", "N", 2, 2, 4, 2, 3, 2), 10), (("N", "N", 4, 2, 2, 4, 2, 3), 13),
+        (("T", "N", 2, 2, 4, 3, 4, 2), 8), (("N", "T", 2, 4, 2, 2, 3, 2), 10),
+    ]
+    for (ta, tb, m, n, k, lda, ldb, ldc), info in cases:
+        C = np.full(32, 7.0)
+        got = oracle.dgemm_cm(ta, tb, m, n, k, 1.0, A, lda, B, ldb, 0.0, C, ldc, "oracle")
+        assert got == info
+        Cr = np.full(32, 7.0)
+        oracle.dgemm_cm(ta, tb, m, n, k, 1.0, A, lda, B, ldb, 0.0, Cr, ldc, "ref")
+        assert np.all(Cr == 7.0)  # XERBLA is silent: C untouched
+        assert np.all(C == 7.0)
+
+
+def test_dgemm_quick_returns(oracle):
+    need_ref(oracle)
+    A = np.full(9, np.nan)
+    B = np.ones(9)
+    for which in ("oracle", "ref"):
+        C = np.arange(9.0)
+        oracle.dgemm_cm("N", "N", 3, 3, 3, 0.0, A, 3, B, 3, 1.0, C, 3, which)
+        assert np.array_equal(C, np.arange(9.0))  # alpha = 0, beta = 1: untouched
+        oracle.dgemm_cm("N", "N", 3, 3, 3, 0.0, A, 3, B, 3, 2.0, C, 3, which)
+        assert np.array_equal(C, 2 * np.arange(9.0))  # A never read
+        oracle.dgemm_cm("N", "N", 3, 3, 3, 0.0, A, 3, B, 3, 0.0, C, 3, which)
+        assert np.array_equal(C, np.zeros(9))
+
+
+# ------------------------------------------------------------------ overlap
+def _bruteforce_overlap(psi, nkx0):
+    x = psi.astype(np.longdouble)
+    w = metric_weights(psi.shape[1] // 2, nkx0).astype(np.longdouble)
+    return (x * w) @ x.T
+
+
+def test_overlap_bruteforce_longdouble(oracle):
+    rng = np.random.default_rng(1)
+    n, npw = 9, 70
+    psi = rng.standard_normal((n, 2 * npw))
+    for nkx0 in (0, 1, 5):
+        S = oracle.overlap(psi, nkx0=nkx0)
+        ref = np.array(_bruteforce_overlap(psi, nkx0), dtype=np.float64)
+        assert np.allclose(S, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+        assert np.array_equal(S, S.T)  # exactly symmetric
+    # a dropped kx0 correction is caught
+    assert not np.allclose(oracle.overlap(psi, nkx0=0), _bruteforce_overlap(psi, 1).astype(float),
+                           atol=1e-6)
+
+
+def test_overlap_plane_split_invariant(oracle):
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal((12, 2 * 257))
+    S1 = oracle.overlap(psi, 1, nplanes=1)
+    S7 = oracle.overlap(psi, 1, nplanes=7)
+    assert np.allclose(S1, S7, rtol=0, atol=1e-13 * np.abs(S1).max())
+
+
+def test_overlap_g0_term_equals_full_sphere(oracle):
+    """Half sphere {G=0} U {+G} with the 2Re - G0 weight equals the complex
+    overlap of the Hermitian-symmetric full sphere (psi(-G) = psi(G)^*)."""
+    rng = np.random.default_rng(3)
+    n, npw = 6, 40
+    half = rng.standard_normal((n, npw)) + 1j * rng.standard_normal((n, npw))
+    half[:, 0] = half[:, 0].real  # G = 0 real
+    full = np.concatenate([half, np.conj(half[:, 1:])], axis=1)  # G=0, +G, -G
+    S_full = (np.conj(full) @ full.T)
+    assert np.allclose(S_full.imag, 0, atol=1e-12)
+    psi = np.empty((n, 2 * npw))
+    psi[:, 0::2], psi[:, 1::2] = half.real, half.imag
+    S = oracle.overlap(psi, nkx0=1)
+    assert np.allclose(S, S_full.real, rtol=1e-13, atol=1e-12)
+
+
+def test_overlap_orthonormal_block_is_identity(oracle):
+    psi = make_states(24, 400, eps=0.0, seed=4).numpy()
+    S = oracle.overlap(psi, nkx0=1)
+    assert np.abs(S - np.eye(24)).max() < 1e-13
+
+
+# ------------------------------------------------------------------ S -> T
+def _closed_form_T(S, nspin):
+    B = S / 2.0 if nspin == 1 else S
+    lam, V = np.linalg.eigh(B)
+    return (V * lam ** -0.5) @ V.T
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_s_to_t_closed_form(oracle, nspin):
+    psi = make_states(48, 600, eps=1e-2, seed=6).numpy()
+    S = oracle.overlap(psi, 1)
+    if nspin == 1:
+        S = 2.0 * S  # occupation-2 convention: S ~ 2I
+    T, it, err, st = oracle.s_to_t(S, nspin=nspin, tol=1e-14, maxit=100)
+    assert st == 0 and err <= 1e-14
+    Tc = _closed_form_T(S, nspin)
+    assert np.abs(T - Tc).max() < 1e-13
+    B = S / 2.0 if nspin == 1 else S
+    assert np.abs(T @ B @ T - np.eye(48)).max() < 1e-13  # T B T = I
+    assert np.abs(T - T.T).max() < 1e-14
+    assert np.abs(T @ S - S @ T).max() < 1e-13
+
+
+def test_s_to_t_special_cases(oracle):
+    n = 16
+    T, it, err, st = oracle.s_to_t(2.0 * np.eye(n), nspin=1, tol=1e-15, maxit=50)
+    assert st == 0 and it == 1 and err == 0.0 and np.array_equal(T, np.eye(n))
+    d = 1.0 + 0.05 * np.linspace(-1, 1, n)
+    T, it, err, st = oracle.s_to_t(2.0 * np.diag(d), nspin=1, tol=1e-15, maxit=50)
+    assert st == 0
+    assert np.allclose(np.diag(T), d ** -0.5, rtol=0, atol=1e-15)
+    assert np.abs(T - np.diag(np.diag(T))).max() == 0.0
+    # lsda: B = S
+    T, it, err, st = oracle.s_to_t(np.diag(d), nspin=2, tol=1e-15, maxit=50)
+    assert np.allclose(np.diag(T), d ** -0.5, rtol=0, atol=1e-15)
+
+
+def test_s_to_t_stop_rule(oracle):
+    """ortho.C:118-133: continue while err > tol and it < maxit; abort status
+    whenever the count reached maxit at exit, even on a converged iterate."""
+    psi = make_states(20, 300, eps=1e-2, seed=8).numpy()
+    S = oracle.overlap(psi, 1)
+    T, it, err, st = oracle.s_to_t(S, nspin=2, tol=1e-15, maxit=100)
+    assert st == 0 and it < 100
+    T2, it2, err2, st2 = oracle.s_to_t(S, nspin=2, tol=1e-15, maxit=it)
+    assert it2 == it and st2 == 1 and np.array_equal(T, T2)
+    T3, it3, err3, st3 = oracle.s_to_t(S, nspin=2, tol=1e-15, maxit=2)
+    assert it3 == 2 and st3 == 1 and err3 > 1e-15
+
+
+def test_piny_iteration_matches_ortho(oracle):
+    psi = make_states(70, 500, eps=1e-3, seed=9).numpy()
+    S = 2.0 * oracle.overlap(psi, 1)
+    Tp, itp, tolp, stp = oracle.piny_iter_tmat(S)
+    To, ito, erro, sto = oracle.s_to_t(S, nspin=1, tol=1e-15, maxit=10)
+    assert stp == 0
+    assert np.abs(Tp - To).max() < 1e-14
+    assert abs(itp - ito) <= 1
+
+
+# ------------------------------------------------------------------ backward
+def test_backmul_linear_map(oracle):
+    rng = np.random.default_rng(10)
+    n, K = 11, 2 * 97
+    M = rng.standard_normal((n, n))
+    psi = rng.standard_normal((n, K))
+    out0 = rng.standard_normal((n, K))
+    out = oracle.backmul(M, psi, transT=False, alpha=0.5, beta=-1.0, out=out0.copy())
+    assert np.allclose(out, 0.5 * M @ psi - out0, rtol=1e-13, atol=1e-13)
+    out = oracle.backmul(M, psi, transT=True, alpha=1.0, beta=0.0, nplanes=3)
+    assert np.allclose(out, M.T @ psi, rtol=1e-13, atol=1e-13)
+    assert not np.allclose(out, M @ psi, atol=1e-3)  # a transposed operand is caught
+
+
+# ------------------------------------------------------------------ whole step
+def test_step_oracle_equals_reference_bitwise(oracle):
+    need_ref(oracle)
+    psi = make_states(33, 257, eps=1e-3, seed=14).numpy()
+    a = oracle.ortho_step(psi, nkx0=1, nspin=2, gemm="oracle")
+    b = oracle.ortho_step(psi, nkx0=1, nspin=2, gemm="ref")
+    for k in ("S", "T", "psi_out"):
+        assert np.array_equal(a[k], b[k])
+    assert a["iters"] == b["iters"]
+
+
+def test_config_a_step_orthonormalises(oracle):
+    """Config A (n=128, npw=8192, eps=1e-3, seed 0): one step leaves
+    ||S(T psi) - I||_max <= 1e-12 (lsda convention, S ~ I)."""
+    psi = make_states(128, 8192, eps=1e-3, seed=0).numpy()
+    r = oracle.ortho_step(psi, nkx0=1, nspin=2, tol=1e-12, maxit=1000, nplanes=8)
+    assert r["status"] == 0
+    res0 = oracle.orthonormality_residual(psi, 1, occ=1.0, nplanes=8)
+    res1 = oracle.orthonormality_residual(r["psi_out"], 1, occ=1.0, nplanes=8)
+    assert 1e-4 < res0 < 1e-2
+    assert res1 < 1e-12
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))))
+def test_golden_fixtures(oracle, path):
+    g = np.load(path)
+    psi = g["psi"]
+    nkx0 = int(g["nkx0"])
+    for nspin in (1, 2):
+        r = oracle.ortho_step(psi, nkx0=nkx0, nspin=nspin, gemm="oracle")
+        assert np.array_equal(r["S"], g[f"S_nspin{nspin}"])
+        assert np.array_equal(r["T"], g[f"T_nspin{nspin}"])
+        assert np.array_equal(r["psi_out"], g[f"psi_out_nspin{nspin}"])
+        assert r["iters"] == int(g[f"iters_nspin{nspin}"])
+    L = oracle.lambda_fwd(psi, g["F"])
+    assert np.array_equal(L, g["L"])
+    assert np.array_equal(oracle.backmul(L, psi, transT=True), g["LT_psi"])
