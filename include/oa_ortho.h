@@ -75,7 +75,9 @@ typedef struct oa_ns_report {
  * No counterpart in the reference (a Charm++ PE's BLAS has no handle). */
 int oa_create(oa_handle_t *h, int device);
 int oa_destroy(oa_handle_t h);
-/* Replace the handle's stream (a hipStream_t); NULL restores the handle's own. */
+/* Run the handle's work on `stream` (a hipStream_t); NULL selects the
+ * device's default (null) stream, as everywhere in HIP.  A new handle starts
+ * on a non-blocking stream of its own. */
 int oa_set_stream(oa_handle_t h, void *stream);
 void *oa_get_stream(oa_handle_t h);
 /* Block until the handle's stream is idle. */

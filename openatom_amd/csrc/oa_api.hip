@@ -179,7 +179,7 @@ int oa_destroy(oa_handle_t c) {
 
 int oa_set_stream(oa_handle_t c, void *stream) {
   if (!c) return 1;
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  c->stream = (hipStream_t)stream;  // NULL is the device's default stream
   return OA_SUCCESS;
 }
 

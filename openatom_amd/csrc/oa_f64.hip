@@ -11,6 +11,8 @@
 //
 // Tiling: 256-thread workgroups (4 wave64s in a 2x2 grid), each wave owns a
 // (16*WM) x (16*WM) block of v_mfma_f64_16x16x4f64 accumulators.  Operand
+// lanes: lane l supplies A[l%16][l/16] and B[l/16][l%16]; result register v
+// of lane l holds D[l/16 + 4v][l%16] (measured on gfx950).  Operand
 // panels of BK=16 doubles are staged through LDS with a row stride of 18
 // doubles, which makes the 16-byte fragment reads of 16 lanes cover all 64
 // banks once.  Each lane reads two consecutive k per LDS access and feeds
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch gb, const int32_
     for (int j = 0; j < WM; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int m = m0 + xrow0 + i * 16 + 4 * g + v;
+        const int m = m0 + xrow0 + i * 16 + g + 4 * v;
         const int n = n0 + yrow0 + j * 16 + r;
         if (m < P.M && n < P.N) {
           double *c = P.C + (long)m * P.ldc + n;
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(NT) void ovl_f64_kernel(OvlArgs a) {
     for (int j = 0; j < WM; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int lm = xrow0 + i * 16 + 4 * g + v;
+        const int lm = xrow0 + i * 16 + g + 4 * v;
         const int ln = yrow0 + j * 16 + r;
         pt[lm * BM + ln] = acc[i][j][v];
       }

@@ -31,10 +31,15 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
 
 
+def _ld(t: torch.Tensor) -> int:
+    """Row pitch in elements (torch reports arbitrary strides for 1-row views)."""
+    return t.stride(0) if t.shape[0] > 1 else t.shape[1]
+
+
 def _need(t: torch.Tensor, name: str, rows: int | None = None) -> None:
     if not t.is_cuda or t.dtype != torch.float64:
         raise TypeError(f"{name} must be a float64 device tensor")
-    if t.dim() != 2 or t.stride(1) != 1:
+    if t.dim() != 2 or (t.stride(1) != 1 and t.shape[1] > 1):
         raise ValueError(f"{name} must be 2-D with unit column stride")
     if rows is not None and t.shape[0] != rows:
         raise ValueError(f"{name} must have {rows} rows")
@@ -90,7 +95,7 @@ class Handle:
         npw = psi.shape[1] // 2
         if S is None:
             S = torch.empty((n, n), dtype=torch.float64, device=psi.device)
-        check(_lib.lib().oa_overlap_sym(self.h, n, npw, _ptr(psi), psi.stride(0), nkx0, alpha,
+        check(_lib.lib().oa_overlap_sym(self.h, n, npw, _ptr(psi), _ld(psi), nkx0, alpha,
                                         prec_code(prec), _ptr(S), diag, _ptr(maxdev)),
               "oa_overlap_sym")
         return S
@@ -103,8 +108,8 @@ class Handle:
         npw = psi.shape[1] // 2
         if L is None:
             L = torch.empty((n, n), dtype=torch.float64, device=psi.device)
-        check(_lib.lib().oa_lambda(self.h, n, npw, _ptr(psi), psi.stride(0), _ptr(F),
-                                   F.stride(0), alpha, prec_code(prec), _ptr(L)), "oa_lambda")
+        check(_lib.lib().oa_lambda(self.h, n, npw, _ptr(psi), _ld(psi), _ptr(F),
+                                   _ld(F), alpha, prec_code(prec), _ptr(L)), "oa_lambda")
         return L
 
     def s_to_t(self, S: torch.Tensor, nspin: int = 1, tol: float = 1e-15, maxit: int = 1000,
@@ -141,7 +146,7 @@ class Handle:
         if out is None:
             out = torch.zeros_like(psi) if beta != 0.0 else torch.empty_like(psi)
         check(_lib.lib().oa_backmul(self.h, n, npw, _ptr(M), 1 if transM else 0, _ptr(psi),
-                                    psi.stride(0), alpha, beta, _ptr(out), out.stride(0),
+                                    _ld(psi), alpha, beta, _ptr(out), _ld(out),
                                     prec_code(prec)), "oa_backmul")
         return out
 
@@ -159,9 +164,9 @@ class Handle:
         out = torch.empty_like(psi) if out is None else out
         it = ctypes.c_int(0)
         err = ctypes.c_double(0)
-        st = _lib.lib().oa_ortho_step(self.h, n, npw, _ptr(psi), psi.stride(0), nkx0, nspin,
+        st = _lib.lib().oa_ortho_step(self.h, n, npw, _ptr(psi), _ld(psi), nkx0, nspin,
                                       tol, maxit, prec_code(prec), _ptr(S), _ptr(T), _ptr(out),
-                                      out.stride(0), ctypes.byref(it) if sync else None,
+                                      _ld(out), ctypes.byref(it) if sync else None,
                                       ctypes.byref(err) if sync else None)
         check(st, "oa_ortho_step")
         if sync:

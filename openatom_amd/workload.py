@@ -53,7 +53,9 @@ def _cholqr2(Z: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
     for _ in range(2):
         S = (Z * w) @ Z.T
         L = torch.linalg.cholesky(S)
-        Z = torch.linalg.solve_triangular(L, Z, upper=False)
+        eye = torch.eye(L.shape[0], dtype=L.dtype, device=L.device)
+        Linv = torch.linalg.solve_triangular(L, eye, upper=False)
+        Z = Linv @ Z
     return Z
 
 
