@@ -49,7 +49,22 @@ def parse():
     ap.add_argument("--maxit", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy of known size (1 GiB read + 1 GiB written per launch),
+    the reference point that scales FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 8,
+                      "bytes_written": b.numel() * 8, "launches": 3}), flush=True)
 
 
 def run_mode(h, psi, prec, args, n, npw):
@@ -149,6 +164,9 @@ def cpu_baseline(n, npw, iters, threads):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share only (--gpus {args.gpus} requested); the "
               "multi-GPU sharding is a design note, not a run.", file=sys.stderr)

@@ -38,47 +38,63 @@ __device__ __forceinline__ int xcd_logical(int b, int nb) {
   return (b & 7) * per + (b >> 3);
 }
 
-// BR x BK operand panel -> registers.  KC: element (r, k) is p[r*ld + k]
-// (k contiguous); otherwise p[k*ld + r] (r contiguous).  Out-of-range
-// elements read as zero.
+// Per-thread loader of one BR x BK operand panel, 16 bytes per load.
+//   KC  (element (r, k) at p[r*ld + k], k contiguous): a row holds 8 chunks;
+//       thread t owns chunk (t % 8) of rows t/8 + 32j.
+//   !KC (element (r, k) at p[k*ld + r], r contiguous): a k-row holds BR/2
+//       chunks; thread t owns chunk (t % (BR/2)) of k-rows t/(BR/2) + NT/(BR/2)*j.
+// load_fast() needs a 16-byte aligned, fully in-range panel; load_masked()
+// reads element by element and zero-fills outside [rlim) x [klim).
 template <int BR, bool KC>
-__device__ __forceinline__ void panel_load(const double *__restrict__ p, long ld,
-                                           int r0, int rlim, int k0, int klim,
-                                           double (&v)[BR * BK / NT], int tid) {
-  constexpr int CNT = BR * BK / NT;
-  if (KC) {
-    const int kk = tid % BK, rr = tid / BK;
-    const int k = k0 + kk;
-#pragma unroll
-    for (int i = 0; i < CNT; i++) {
-      const int r = r0 + rr + i * (NT / BK);
-      v[i] = (r < rlim && k < klim) ? p[(long)r * ld + k] : 0.0;
-    }
-  } else {
-    const int rr = tid % BR, kk = tid / BR;
-    const int r = r0 + rr;
-#pragma unroll
-    for (int i = 0; i < CNT; i++) {
-      const int k = k0 + kk + i * (NT / BR);
-      v[i] = (r < rlim && k < klim) ? p[(long)k * ld + r] : 0.0;
-    }
-  }
-}
+struct PanelLoader {
+  static constexpr int CH = BR * BK / 2 / NT;
+  static constexpr int KSTEP = KC ? 0 : NT / (BR / 2);
+  const double *base;  // matrix origin
+  long ld;
+  int rb, kb;          // this thread's first (row, k) inside the panel
+  d2v v[CH];
 
-template <int BR, bool KC>
-__device__ __forceinline__ void panel_store(double *s, const double (&v)[BR * BK / NT],
-                                            int tid) {
-  constexpr int CNT = BR * BK / NT;
-  if (KC) {
-    const int kk = tid % BK, rr = tid / BK;
-#pragma unroll
-    for (int i = 0; i < CNT; i++) s[(rr + i * (NT / BK)) * LS + kk] = v[i];
-  } else {
-    const int rr = tid % BR, kk = tid / BR;
-#pragma unroll
-    for (int i = 0; i < CNT; i++) s[rr * LS + kk + i * (NT / BR)] = v[i];
+  __device__ __forceinline__ void init(const double *p, long ld_, int tid) {
+    base = p;
+    ld = ld_;
+    if (KC) { rb = tid / 8; kb = (tid % 8) * 2; }
+    else { rb = (tid % (BR / 2)) * 2; kb = tid / (BR / 2); }
   }
-}
+  __device__ __forceinline__ void load_fast(int r0, int k0) {
+    if (KC) {
+      const double *q = base + (long)(r0 + rb) * ld + k0 + kb;
+#pragma unroll
+      for (int j = 0; j < CH; j++) v[j] = *reinterpret_cast<const d2v *>(q + (long)(32 * j) * ld);
+    } else {
+      const double *q = base + (long)(k0 + kb) * ld + r0 + rb;
+#pragma unroll
+      for (int j = 0; j < CH; j++)
+        v[j] = *reinterpret_cast<const d2v *>(q + (long)(KSTEP * j) * ld);
+    }
+  }
+  __device__ __forceinline__ void load_masked(int r0, int rlim, int k0, int klim) {
+#pragma unroll
+    for (int j = 0; j < CH; j++)
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int r = KC ? r0 + rb + 32 * j : r0 + rb + e;
+        const int k = KC ? k0 + kb + e : k0 + kb + KSTEP * j;
+        v[j][e] = (r < rlim && k < klim) ? base[KC ? (long)r * ld + k : (long)k * ld + r] : 0.0;
+      }
+  }
+  __device__ __forceinline__ void store(double *s) const {
+    if (KC) {
+#pragma unroll
+      for (int j = 0; j < CH; j++) *reinterpret_cast<d2v *>(s + (rb + 32 * j) * LS + kb) = v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < CH; j++) {
+        s[rb * LS + kb + KSTEP * j] = v[j][0];
+        s[(rb + 1) * LS + kb + KSTEP * j] = v[j][1];
+      }
+    }
+  }
+};
 
 // One BK-deep LDS stage of MFMAs for a wave's WM x WM block grid.
 template <int WM>
@@ -104,44 +120,98 @@ __device__ __forceinline__ void mma_stage(const double *Xs, const double *Ys, in
   }
 }
 
-// Main K loop shared by every FP64 kernel: register-prefetched panels, one
-// LDS buffer per operand, two barriers per stage.  When YSAME the Y panel is
-// the X panel (diagonal tile of a symmetric product) and is loaded once.
+// Main K loop shared by every FP64 kernel.  Two LDS buffers per operand and
+// one barrier per BK stage: stage s+1 is fetched into registers while stage s
+// feeds the MFMAs, then written to the other buffer.  `fast` selects 16-byte
+// unmasked loads (interior, aligned tiles); the last stage falls back to
+// masked loads when K is not a multiple of BK.  When YSAME the Y panel is the
+// X panel (diagonal tile of a symmetric product) and is loaded once.
 template <int WM, bool XKC, bool YKC, bool YSAME>
 __device__ __forceinline__ void kloop(const double *X, long ldx, int m0, int mlim,
-                                      const double *Y, long ldy, int n0, int nlim,
-                                      int ka, int kb, double *Xs, double *Ys,
-                                      d4 (&acc)[WM][WM]) {
+                                      const double *Y, long ldy, int n0, int nlim, int ka,
+                                      int kb, bool fast, double *smem, d4 (&acc)[WM][WM]) {
   constexpr int BM = 32 * WM;
+  constexpr int PANEL = BM * LS;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
-  double rx[BM * BK / NT], ry[BM * BK / NT];
-  panel_load<BM, XKC>(X, ldx, m0, mlim, ka, kb, rx, tid);
-  if (!YSAME) panel_load<BM, YKC>(Y, ldy, n0, nlim, ka, kb, ry, tid);
-  for (int k0 = ka; k0 < kb; k0 += BK) {
-    __syncthreads();
-    panel_store<BM, XKC>(Xs, rx, tid);
-    if (!YSAME) panel_store<BM, YKC>(Ys, ry, tid);
-    __syncthreads();
-    if (k0 + BK < kb) {
-      panel_load<BM, XKC>(X, ldx, m0, mlim, k0 + BK, kb, rx, tid);
-      if (!YSAME) panel_load<BM, YKC>(Y, ldy, n0, nlim, k0 + BK, kb, ry, tid);
+  PanelLoader<BM, XKC> lx;
+  PanelLoader<BM, YKC> ly;
+  lx.init(X, ldx, tid);
+  if (!YSAME) ly.init(Y, ldy, tid);
+  const int nst = (kb - ka + BK - 1) / BK;
+  if (nst <= 0) return;
+  auto fetch = [&](int k0) {
+    if (fast && k0 + BK <= kb) {
+      lx.load_fast(m0, k0);
+      if (!YSAME) ly.load_fast(n0, k0);
+    } else {
+      lx.load_masked(m0, mlim, k0, kb);
+      if (!YSAME) ly.load_masked(n0, nlim, k0, kb);
     }
-    mma_stage<WM>(Xs, YSAME ? Xs : Ys, xrow0, yrow0, lane, acc);
+  };
+  // buffers: X at smem + b*PANEL, Y at smem + (2 + b)*PANEL (plain pointer
+  // arithmetic keeps the LDS address space visible to the compiler)
+  fetch(ka);
+  lx.store(smem);
+  if (!YSAME) ly.store(smem + 2 * PANEL);
+  __syncthreads();
+  for (int s = 0; s < nst; s++) {
+    const int cur = (s & 1) * PANEL, nxt = PANEL - cur;
+    const bool more = s + 1 < nst;
+    if (more) fetch(ka + (s + 1) * BK);
+    mma_stage<WM>(smem + cur, YSAME ? smem + cur : smem + 2 * PANEL + cur, xrow0, yrow0, lane,
+                  acc);
+    if (more) {
+      lx.store(smem + nxt);
+      if (!YSAME) ly.store(smem + 2 * PANEL + nxt);
+    }
+    __syncthreads();
   }
 }
 
-struct GemmBatch {
-  oa_gemm_problem p[2];
+__device__ __forceinline__ bool aligned16(const void *p) {
+  return ((uintptr_t)p & 15) == 0;
+}
+
+// Field-by-field selection keeps the problem in scalar registers (indexing
+// the kernel-argument array with blockIdx.y would copy it to scratch).
+struct ProbRegs {
+  int M, N, K;
+  const double *X;
+  long ldx;
+  const double *Y;
+  long ldy;
+  double *C;
+  long ldc;
+  double alpha, beta, diag_add;
 };
 
+__device__ __forceinline__ ProbRegs pick_problem(const oa_gemm_problem &p0,
+                                                 const oa_gemm_problem &p1) {
+  const bool y = blockIdx.y != 0;
+  ProbRegs r;
+  r.M = y ? p1.M : p0.M;
+  r.N = y ? p1.N : p0.N;
+  r.K = y ? p1.K : p0.K;
+  r.X = y ? p1.X : p0.X;
+  r.ldx = y ? p1.ldx : p0.ldx;
+  r.Y = y ? p1.Y : p0.Y;
+  r.ldy = y ? p1.ldy : p0.ldy;
+  r.C = y ? p1.C : p0.C;
+  r.ldc = y ? p1.ldc : p0.ldc;
+  r.alpha = y ? p1.alpha : p0.alpha;
+  r.beta = y ? p1.beta : p0.beta;
+  r.diag_add = y ? p1.diag_add : p0.diag_add;
+  return r;
+}
+
 template <bool TX, bool TY, int WM>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch gb, const int32_t *skip) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void gemm_f64_kernel(oa_gemm_problem p0, oa_gemm_problem p1, const int32_t *skip) {
   if (skip && *skip) return;
-  const oa_gemm_problem &P = gb.p[blockIdx.y];
+  const ProbRegs P = pick_problem(p0, p1);
   constexpr int BM = 32 * WM;
-  __shared__ __attribute__((aligned(16))) double Xs[BM * LS];
-  __shared__ __attribute__((aligned(16))) double Ys[BM * LS];
+  __shared__ __attribute__((aligned(16))) double smem[4 * BM * LS];
   const int tmn = cdiv(P.M, BM), tnn = cdiv(P.N, BM);
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   if (logical >= tmn * tnn) return;
@@ -152,27 +222,44 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmBatch gb, const int32_
   for (int i = 0; i < WM; i++)
 #pragma unroll
     for (int j = 0; j < WM; j++) acc[i][j] = d4{0, 0, 0, 0};
-  kloop<WM, !TX, TY, false>(P.X, P.ldx, m0, P.M, P.Y, P.ldy, n0, P.N, 0, P.K, Xs, Ys, acc);
+  const bool fast = m0 + BM <= P.M && n0 + BM <= P.N && aligned16(P.X) && aligned16(P.Y) &&
+                    (P.ldx % 2) == 0 && (P.ldy % 2) == 0;
+  kloop<WM, !TX, TY, false>(P.X, P.ldx, m0, P.M, P.Y, P.ldy, n0, P.N, 0, P.K, fast, smem, acc);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
   const int r = lane & 15, g = lane >> 4;
+  const double alpha = P.alpha, beta = P.beta, dadd = P.diag_add;
+  if (beta == 0.0) {
 #pragma unroll
-  for (int i = 0; i < WM; i++)
-#pragma unroll
-    for (int j = 0; j < WM; j++)
+    for (int i = 0; i < WM; i++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int m = m0 + xrow0 + i * 16 + g + 4 * v;
-        const int n = n0 + yrow0 + j * 16 + r;
-        if (m < P.M && n < P.N) {
-          double *c = P.C + (long)m * P.ldc + n;
-          double val = P.alpha * acc[i][j][v];
-          if (P.beta != 0.0) val += P.beta * (*c);
-          if (m == n && P.diag_add != 0.0) val += P.diag_add;
-          *c = val;
+        if (m >= P.M) continue;
+        double *crow = P.C + (long)m * P.ldc;
+#pragma unroll
+        for (int j = 0; j < WM; j++) {
+          const int n = n0 + yrow0 + j * 16 + r;
+          if (n < P.N) crow[n] = alpha * acc[i][j][v] + ((m == n) ? dadd : 0.0);
         }
       }
+  } else {
+#pragma unroll
+    for (int i = 0; i < WM; i++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = m0 + xrow0 + i * 16 + g + 4 * v;
+        if (m >= P.M) continue;
+        double *crow = P.C + (long)m * P.ldc;
+#pragma unroll
+        for (int j = 0; j < WM; j++) {
+          const int n = n0 + yrow0 + j * 16 + r;
+          if (n < P.N)
+            crow[n] = alpha * acc[i][j][v] + beta * crow[n] + ((m == n) ? dadd : 0.0);
+        }
+      }
+  }
 }
 
 // Split-K partial products over the plane-wave index.
@@ -200,10 +287,10 @@ __device__ __forceinline__ void upper_tile(int tile, int T, int &bi, int &bj) {
 }
 
 template <int WM, bool SYM>
-__global__ __launch_bounds__(NT) void ovl_f64_kernel(OvlArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void ovl_f64_kernel(OvlArgs a) {
   constexpr int BM = 32 * WM;
-  __shared__ __attribute__((aligned(16))) double Xs[BM * LS];
-  __shared__ __attribute__((aligned(16))) double Ys[BM * LS];
+  __shared__ __attribute__((aligned(16))) double smem[4 * BM * LS];
   const int Tm = cdiv(a.M, BM), Tn = cdiv(a.N, BM);
   const int ntiles = SYM ? Tm * (Tm + 1) / 2 : Tm * Tn;
   const int nz = a.nz_main + (a.kx0_end > 0 ? 1 : 0);
@@ -227,12 +314,17 @@ __global__ __launch_bounds__(NT) void ovl_f64_kernel(OvlArgs a) {
 #pragma unroll
     for (int j = 0; j < WM; j++) acc[i][j] = d4{0, 0, 0, 0};
   const int m0 = bi * BM, n0 = bj * BM;
+  const double *Yp = SYM ? a.X : a.Y;
+  const long ldy = SYM ? a.ldx : a.ldy;
+  const bool fast = m0 + BM <= a.M && n0 + BM <= a.N && aligned16(a.X) && aligned16(Yp) &&
+                    (a.ldx % 2) == 0 && (ldy % 2) == 0 && (ka % 2) == 0;
   if (ka < kb) {
     if (SYM && bi == bj)
-      kloop<WM, true, true, true>(a.X, a.ldx, m0, a.M, a.X, a.ldx, n0, a.N, ka, kb, Xs, Ys, acc);
+      kloop<WM, true, true, true>(a.X, a.ldx, m0, a.M, a.X, a.ldx, n0, a.N, ka, kb, fast, smem,
+                                  acc);
     else
-      kloop<WM, true, true, false>(a.X, a.ldx, m0, a.M, SYM ? a.X : a.Y, SYM ? a.ldx : a.ldy,
-                                   n0, a.N, ka, kb, Xs, Ys, acc);
+      kloop<WM, true, true, false>(a.X, a.ldx, m0, a.M, Yp, ldy, n0, a.N, ka, kb, fast, smem,
+                                   acc);
   }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
@@ -367,17 +459,18 @@ int launch_gemm(hipStream_t s, const oa_gemm_problem *p, int nprob, const int32_
   int tiles128 = 0;
   for (int i = 0; i < nprob; i++)
     tiles128 = max(tiles128, cdiv(p[i].M, 128) * cdiv(p[i].N, 128));
-  GemmBatch gb;
-  for (int i = 0; i < nprob; i++) gb.p[i] = p[i];
+  const oa_gemm_problem p0 = p[0], p1 = p[nprob > 1 ? 1 : 0];
   if (tiles128 * nprob >= 512) {
     const int nb = cdiv(tiles128, 8) * 8;
-    hipLaunchKernelGGL((gemm_f64_kernel<TX, TY, 4>), dim3(nb, nprob), dim3(NT), 0, s, gb, skip);
+    hipLaunchKernelGGL((gemm_f64_kernel<TX, TY, 4>), dim3(nb, nprob), dim3(NT), 0, s, p0, p1,
+                       skip);
   } else {
     int tiles64 = 0;
     for (int i = 0; i < nprob; i++)
       tiles64 = max(tiles64, cdiv(p[i].M, 64) * cdiv(p[i].N, 64));
     const int nb = cdiv(tiles64, 8) * 8;
-    hipLaunchKernelGGL((gemm_f64_kernel<TX, TY, 2>), dim3(nb, nprob), dim3(NT), 0, s, gb, skip);
+    hipLaunchKernelGGL((gemm_f64_kernel<TX, TY, 2>), dim3(nb, nprob), dim3(NT), 0, s, p0, p1,
+                       skip);
   }
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
