@@ -50,13 +50,21 @@ extern "C" {
  *   OA_PREC_BF16      Ozaki digit slices on bf16 MFMA, exact fp32 sums.
  *   OA_PREC_FP32      Ozaki digit slices on fp32 MFMA, exact fp32 sums.
  * For the slice modes, OA_PREC_SLICES(p, s) selects s digit slices
- * (2 <= s <= 6); the default is 3. */
+ * (2 <= s <= 6); the default is 3.  The overlap keeps the products of
+ * levels p + q <= s + 1.  The back-multiply splits T = D + X, applies D in
+ * FP64 to psi rebuilt from all s digits and keeps levels p + q <= s of the
+ * X * psi digit products, or p + q <= s - 1 with OA_PREC_LEAN_BACKMUL. */
 #define OA_PREC_FP64 0
 #define OA_PREC_INT8 1
 #define OA_PREC_FP16 2
 #define OA_PREC_BF16 3
 #define OA_PREC_FP32 4
 #define OA_PREC_SLICES(p, s) ((p) | ((s) << 8))
+#define OA_PREC_LEAN_BACKMUL (1 << 12)
+/* oa_backmul only: reuse the digit slices of psi left in the handle's
+ * workspace by the preceding oa_overlap_sym of the same psi and precision
+ * (the caller guarantees psi is unchanged in between). */
+#define OA_PREC_REUSE_DIGITS (1 << 16)
 
 typedef struct oa_context *oa_handle_t;
 
@@ -85,6 +93,8 @@ int oa_synchronize(oa_handle_t h);
 /* Pre-size the handle's workspace for instances up to (n, npw) so that later
  * calls allocate nothing (needed before hipGraph capture). */
 int oa_reserve(oa_handle_t h, int n, int npw);
+/* Same, including the digit-slice workspace of a split precision `prec`. */
+int oa_reserve_prec(oa_handle_t h, int n, int npw, int prec);
 const char *oa_status_string(int status);
 
 /* ---- BLAS drop-ins (Fortran ABI) --------------------------------------- */

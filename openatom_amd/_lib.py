@@ -25,22 +25,25 @@ OA_PREC_INT8 = 1
 OA_PREC_FP16 = 2
 OA_PREC_BF16 = 3
 OA_PREC_FP32 = 4
+OA_PREC_REUSE_DIGITS = 1 << 16
+OA_PREC_LEAN_BACKMUL = 1 << 12
 
 
 def prec_code(prec) -> int:
-    """'fp64' | 'int8x5' | 'fp16x3' | 'bf16x4' | 'fp32x3' | int -> C precision code."""
+    """'fp64' | 'int8x5' | 'int8x6l' | 'fp16x3' | 'bf16x4' | int -> C precision code.
+    A trailing 'l' selects the lean back-multiply (one digit level fewer)."""
     if isinstance(prec, int):
         return prec
     p = prec.lower()
     if p == "fp64":
         return OA_PREC_FP64
-    m = re.fullmatch(r"(int8|fp16|bf16|fp32)(?:x(\d))?", p)
+    m = re.fullmatch(r"(int8|fp16|bf16|fp32)(?:x(\d))?(l?)", p)
     if not m:
         raise ValueError(f"unknown precision {prec!r}")
     base = {"int8": OA_PREC_INT8, "fp16": OA_PREC_FP16, "bf16": OA_PREC_BF16,
             "fp32": OA_PREC_FP32}[m.group(1)]
     slices = int(m.group(2) or 3)
-    return base | (slices << 8)
+    return base | (slices << 8) | (OA_PREC_LEAN_BACKMUL if m.group(3) else 0)
 
 
 class OaError(RuntimeError):
@@ -86,6 +89,7 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_get_stream, _vp, _vp)
     _sig(L.oa_synchronize, _i, _vp)
     _sig(L.oa_reserve, _i, _vp, _i, _i)
+    _sig(L.oa_reserve_prec, _i, _vp, _i, _i, _i)
     _sig(L.oa_status_string, ctypes.c_char_p, _i)
     _sig(L.oa_dgemm, _i, _vp, ctypes.c_char, ctypes.c_char, _i, _i, _i, _d, _vp, _i, _vp,
          _i, _d, _vp, _i)

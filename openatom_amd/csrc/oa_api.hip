@@ -106,19 +106,21 @@ size_t span_elems(int ld, int rows, int cols) {
 
 }  // namespace
 
-int oa_ws_require(oa_context *c, size_t bytes) {
-  if (bytes <= c->ws.bytes) return OA_SUCCESS;
+int oa_ws_require_into(oa_context *c, oa_workspace &w, size_t bytes) {
+  if (bytes <= w.bytes) return OA_SUCCESS;
   if (is_capturing(c->stream)) return OA_ERR_CAPACITY;  // no allocation under capture
   OA_HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (c->ws.ptr) OA_HIP_CHECK(hipFree(c->ws.ptr));
-  c->ws.ptr = nullptr;
-  c->ws.bytes = 0;
+  if (w.ptr) OA_HIP_CHECK(hipFree(w.ptr));
+  w.ptr = nullptr;
+  w.bytes = 0;
   const size_t gran = (size_t)1 << 24;
   const size_t sz = (bytes + gran - 1) / gran * gran;
-  OA_HIP_CHECK(hipMalloc(&c->ws.ptr, sz));
-  c->ws.bytes = sz;
+  OA_HIP_CHECK(hipMalloc(&w.ptr, sz));
+  w.bytes = sz;
   return OA_SUCCESS;
 }
+
+int oa_ws_require(oa_context *c, size_t bytes) { return oa_ws_require_into(c, c->ws, bytes); }
 
 extern "C" {
 
@@ -169,6 +171,7 @@ int oa_destroy(oa_handle_t c) {
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   if (c->stream && c->stream != c->own_stream) (void)hipStreamSynchronize(c->stream);
   if (c->ws.ptr) (void)hipFree(c->ws.ptr);
+  if (c->ns.ptr) (void)hipFree(c->ns.ptr);
   if (c->report) (void)hipFree(c->report);
   if (c->host_report) (void)hipHostFree(c->host_report);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -197,8 +200,6 @@ static size_t ws_for(int n, int npw) {
   size_t b = oa_overlap_f64_ws_bytes(n, K);
   const size_t l = oa_ntsplit_f64_ws_bytes(n, n, K);
   if (l > b) b = l;
-  const size_t ns = oa_ns_ws_bytes(n);
-  if (ns > b) b = ns;
   return b;
 }
 
@@ -207,7 +208,19 @@ int oa_reserve(oa_handle_t c, int n, int npw) {
   if (n < 0) return 2;
   if (npw < 0) return 3;
   DeviceGuard g(c->device);
-  return oa_ws_require(c, ws_for(n, npw));
+  const int st = oa_ws_require(c, ws_for(n, npw));
+  if (st) return st;
+  return oa_ws_require_into(c, c->ns, oa_ns_ws_bytes(n));
+}
+
+int oa_reserve_prec(oa_handle_t c, int n, int npw, int prec) {
+  if (!c) return 1;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  int st = oa_reserve(c, n, npw);
+  if (st) return st;
+  DeviceGuard g(c->device);
+  return oa_ws_require(c, oa_split_ws_bytes(prec, n, 2 * npw));
 }
 
 // ------------------------------------------------------------- BLAS drop-ins
@@ -358,9 +371,9 @@ int oa_s_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int 
   if (n > 0 && !T) return 7;
   if (n == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
-  int st = oa_ws_require(c, oa_ns_ws_bytes(n));
+  int st = oa_ws_require_into(c, c->ns, oa_ns_ws_bytes(n));
   if (st) return st;
-  return s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ws.ptr);
+  return s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ns.ptr);
 }
 
 int oa_s_to_t_async(oa_handle_t c, int n, const double *S, int nspin, double tol, int maxit,
@@ -374,9 +387,9 @@ int oa_s_to_t_async(oa_handle_t c, int n, const double *S, int nspin, double tol
   if (!report) return 8;
   if (n == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
-  int st = oa_ws_require(c, oa_ns_ws_bytes(n));
+  int st = oa_ws_require_into(c, c->ns, oa_ns_ws_bytes(n));
   if (st) return st;
-  return oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, report, c->ws.ptr);
+  return oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, report, c->ns.ptr);
 }
 
 int oa_backmul(oa_handle_t c, int n, int npw, const double *M, int transM, const double *psi,
@@ -393,7 +406,7 @@ int oa_backmul(oa_handle_t c, int n, int npw, const double *M, int transM, const
   if ((prec & 0xff) != OA_PREC_FP64) {
     int st = oa_ws_require(c, oa_split_ws_bytes(prec, n, K));
     if (st) return st;
-    return oa_launch_backmul_split(c->stream, prec, n, K, M, transM, psi, ldpsi, alpha, beta,
+    return oa_launch_backmul_split(c->stream, prec, n, K, M, transM, psi, ldpsi, 0, alpha, beta,
                                    out, ldout, c->ws.ptr, c->ws.bytes);
   }
   oa_gemm_problem p;
@@ -427,21 +440,21 @@ int oa_ortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, 
   if (ldout < 2L * npw) return 14;
   if (n == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
-  int st = oa_ws_require(c, ws_for(n, npw) > oa_split_ws_bytes(prec, n, 2 * npw)
-                                ? ws_for(n, npw)
-                                : oa_split_ws_bytes(prec, n, 2 * npw));
+  int st = oa_reserve_prec(c, n, npw, prec);
   if (st) return st;
   st = oa_overlap_sym(c, n, npw, psi, ldpsi, nkx0, 2.0, prec, S, 0.0, nullptr);
   if (st) return st;
+  // the split back-multiply reuses the digits the overlap just produced
+  const int bprec = ((prec & 0xff) == OA_PREC_FP64) ? prec : (prec | OA_PREC_REUSE_DIGITS);
   if (iters || err) {
-    const int st_ns = s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ws.ptr);
+    const int st_ns = s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ns.ptr);
     if (st_ns != OA_SUCCESS && st_ns != OA_ERR_NOT_CONVERGED) return st_ns;
-    st = oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, prec);
+    st = oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, bprec);
     return st ? st : st_ns;
   }
-  st = oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, c->report, c->ws.ptr);
+  st = oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, c->report, c->ns.ptr);
   if (st) return st;
-  return oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, prec);
+  return oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, bprec);
 }
 
 }  // extern "C"

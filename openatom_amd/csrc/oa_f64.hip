@@ -480,7 +480,7 @@ struct OvlPlan {
   int WM, BM, ntiles, nz_main, kchunk, nz;
 };
 
-OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym) {
+OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym, int nz_override = 0) {
   OvlPlan pl;
   const int t128 = sym ? cdiv(M, 128) * (cdiv(M, 128) + 1) / 2 : cdiv(M, 128) * cdiv(N, 128);
   pl.WM = (M >= 256 || t128 >= 16) ? 4 : 2;
@@ -488,8 +488,9 @@ OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym) {
   const int Tm = cdiv(M, pl.BM), Tn = cdiv(N, pl.BM);
   pl.ntiles = sym ? Tm * (Tm + 1) / 2 : Tm * Tn;
   const int Kmain = K - kx0_end;
-  // aim for ~1024 workgroups, chunks of at least 8 stages
-  int nz = cdiv(1024, pl.ntiles);
+  // about two waves of the 512 resident workgroups (256 CUs x 2), chunks of
+  // at least 8 stages; an autotuned depth (oa_tune_overlap) takes precedence
+  int nz = nz_override > 0 ? nz_override : cdiv(1024, pl.ntiles);
   nz = min(nz, max(1, cdiv(Kmain, 8 * BK)));
   nz = max(1, min(nz, 512));
   int kchunk = cdiv(cdiv(max(Kmain, 1), nz), BK) * BK;

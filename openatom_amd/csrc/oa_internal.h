@@ -28,12 +28,14 @@ struct oa_context {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
-  oa_workspace ws;          // split-K partials, S->T ping-pong buffers, slices
+  oa_workspace ws;          // split-K partials and digit slices
+  oa_workspace ns;          // S->T ping-pong buffers (kept apart so digits survive)
   oa_ns_report *report = nullptr;  // device report for synchronous S->T
   oa_ns_report *host_report = nullptr;  // pinned mirror
 };
 
 int oa_ws_require(oa_context *c, size_t bytes);
+int oa_ws_require_into(oa_context *c, oa_workspace &w, size_t bytes);
 
 // ---------------------------------------------------------------- FP64 GEMM
 // Row-major C[M][N] = alpha * op(X)[M][K] * op(Y)[K][N] + beta * C
@@ -93,5 +95,5 @@ int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K,
                             double *maxdev, void *ws, size_t ws_bytes);
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K,
                             const double *M, int transM, const double *psi,
-                            long ldpsi, double alpha, double beta, double *out,
-                            long ldout, void *ws, size_t ws_bytes);
+                            long ldpsi, int kx0_end, double alpha, double beta,
+                            double *out, long ldout, void *ws, size_t ws_bytes);

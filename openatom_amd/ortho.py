@@ -62,8 +62,8 @@ class Handle:
         self.stream = s
         check(_lib.lib().oa_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)), "oa_set_stream")
 
-    def reserve(self, n: int, npw: int) -> None:
-        check(_lib.lib().oa_reserve(self.h, n, npw), "oa_reserve")
+    def reserve(self, n: int, npw: int, prec="fp64") -> None:
+        check(_lib.lib().oa_reserve_prec(self.h, n, npw, prec_code(prec)), "oa_reserve_prec")
 
     def synchronize(self) -> None:
         check(_lib.lib().oa_synchronize(self.h), "oa_synchronize")
@@ -138,16 +138,21 @@ class Handle:
 
     def backmul(self, M: torch.Tensor, psi: torch.Tensor, transM: bool = False,
                 alpha: float = 1.0, beta: float = 0.0, out: torch.Tensor | None = None,
-                prec="fp64") -> torch.Tensor:
+                prec="fp64", reuse_digits: bool = False) -> torch.Tensor:
+        """reuse_digits: take psi's digit slices from the preceding overlap_sym
+        of the same psi (split precisions only)."""
         n = psi.shape[0]
         _need(psi, "psi")
         _need(M, "M", n)
         npw = psi.shape[1] // 2
         if out is None:
             out = torch.zeros_like(psi) if beta != 0.0 else torch.empty_like(psi)
+        pc = prec_code(prec)
+        if reuse_digits and (pc & 0xff) != _lib.OA_PREC_FP64:
+            pc |= _lib.OA_PREC_REUSE_DIGITS
         check(_lib.lib().oa_backmul(self.h, n, npw, _ptr(M), 1 if transM else 0, _ptr(psi),
-                                    _ld(psi), alpha, beta, _ptr(out), _ld(out),
-                                    prec_code(prec)), "oa_backmul")
+                                    _ld(psi), alpha, beta, _ptr(out), _ld(out), pc),
+              "oa_backmul")
         return out
 
     def ortho_step(self, psi: torch.Tensor, nkx0: int = 1, nspin: int = 1,
