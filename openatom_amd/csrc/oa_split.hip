@@ -363,36 +363,36 @@ __device__ __forceinline__ void upper_tile_d(int tile, int T, int &bi, int &bj) 
 
 // Level-grouped digit products of one 32-byte stage for a 32 x 64 wave tile:
 // acc[l] += sum over (p, q) with p < SA, q < SB, p + q = l < NL of
-// X_p Y_q^T.  All X slices stay in registers; Y slices stream one at a time.
+// X_p Y_q^T.  Panels hold 128 rows x 32 bytes per slice with the two 16-byte
+// halves of row r swapped when bit 3 of r is set, which spreads any 16
+// consecutive lanes' 16-byte fragment reads over all 64 LDS banks.
 template <int DT, int SA, int SB, int NL>
 __device__ __forceinline__ void digit_stage(const char *Xs, const char *Ys, int xrow, int yrow,
                                             int lane,
                                             typename DTraits<DT>::acc_t (&acc)[NL][2]) {
-  constexpr int ES = DTraits<DT>::ES;
-  constexpr int SK = 32 / ES;               // contraction elements per stage
-  constexpr int ROWB = SK * ES + 16;        // LDS row pitch (bytes)
-  constexpr int PANEL = DBM * ROWB;         // one slice panel
-  constexpr int NSUB = SK / DTraits<DT>::KI;
+  constexpr int PANEL = DBM * 32;
+  const int r = lane & 31;
+  const int boff = r * 32 + 16 * ((lane >> 5) ^ ((r >> 3) & 1));
+  i4v a[SA];
 #pragma unroll
-  for (int sub = 0; sub < NSUB; sub++) {
-    const int boff = (lane >> 5) * 16 + sub * 32;
-    i4v a[SA];
+  for (int p = 0; p < SA; p++) a[p] = *reinterpret_cast<const i4v *>(Xs + p * PANEL + xrow * 32 + boff);
 #pragma unroll
-    for (int p = 0; p < SA; p++)
-      a[p] = *reinterpret_cast<const i4v *>(Xs + p * PANEL + (xrow + (lane & 31)) * ROWB + boff);
+  for (int q = 0; q < SB && q < NL; q++) {
+    i4v b[2];
 #pragma unroll
-    for (int q = 0; q < SB && q < NL; q++) {
-      i4v b[2];
+    for (int j = 0; j < 2; j++)
+      b[j] = *reinterpret_cast<const i4v *>(Ys + q * PANEL + (yrow + j * 32) * 32 + boff);
 #pragma unroll
-      for (int j = 0; j < 2; j++)
-        b[j] = *reinterpret_cast<const i4v *>(Ys + q * PANEL +
-                                              (yrow + j * 32 + (lane & 31)) * ROWB + boff);
+    for (int p = 0; p < SA && p + q < NL; p++)
 #pragma unroll
-      for (int p = 0; p < SA && p + q < NL; p++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc[p + q][j] = mfma_digit<DT>(a[p], b[j], acc[p + q][j]);
-    }
+      for (int j = 0; j < 2; j++) acc[p + q][j] = mfma_digit<DT>(a[p], b[j], acc[p + q][j]);
   }
+}
+
+// s_waitcnt with vmcnt = N and the other counters left at their maximum.
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
 // Digit of DB at element offset idx of a row of 32 (4 consecutive rows per
@@ -428,17 +428,16 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
   typedef typename DTraits<DT>::acc_t ACC;
   constexpr int NL = (MODE == 0) ? S : NLB;  // slices in the product and levels kept
   constexpr int ES = DTraits<DT>::ES;
-  constexpr int SK = 32 / ES;                      // contraction elements per stage
-  constexpr int ROWB = SK * ES + 16;
-  constexpr int PANEL = DBM * ROWB;
-  constexpr int BUF = 2 * NL * PANEL;              // X and Y slices of one stage
-  constexpr int CPR = SK * ES / 16;                // 16-byte chunks per row
-  constexpr int CHUNKS = NL * DBM * CPR;           // per operand per stage
-  constexpr int CPT = (CHUNKS + DNT - 1) / DNT;
+  constexpr int SK = 32 / ES;                 // contraction elements per stage
+  constexpr int PANEL = DBM * 32;             // bytes of one slice panel per stage
+  constexpr int STAGE = 2 * NL * PANEL;       // X and Y slices of one stage
+  constexpr int NBUF = 3;                     // LDS ring depth
+  constexpr int NI = 2 * NL * 4;              // 1 KB wave-loads per stage
+  constexpr int PERW = NI / 8;                // per wave (8 waves)
   // fp32 accumulation is exact only below 2^24 (level sums < 28544 per k for
   // S <= 5): promote every PROMOTE stages into one FP64 sum per output
   constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 512 / SK;
-  __shared__ __attribute__((aligned(16))) char smem[2 * BUF];
+  __shared__ __attribute__((aligned(1024))) char smem[NBUF * STAGE];
 
   int bi, bj, z = 0, tile;
   if (MODE == 0) {
@@ -463,7 +462,6 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
   const int m0 = bi * DBM, n0 = bj * DBM;
   const int ka = (MODE == 0) ? z * a.kc : 0;
   const int kb = (MODE == 0) ? min(a.kfull, ka + a.kc) : a.kfull;
-  const bool ysame = (MODE == 0) && (bi == bj);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xrow = (w >> 1) * 32, yrow = (w & 1) * 64;
 
@@ -480,60 +478,48 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
 
   const char *Xg = reinterpret_cast<const char *>(a.X);
   const char *Yg = reinterpret_cast<const char *>(a.Y);
-  i4v rx[CPT], ry[CPT];
-  auto fetch = [&](int k0) {
+  // Wave-load i of a stage: operand o, slice p, quarter q (rows 32q..32q+31).
+  // Lane L fills LDS slot q*64 + L of the panel, i.e. row r = (q*64+L)/2,
+  // stored half hs = L%2, which holds global half hs ^ ((r >> 3) & 1).
+  auto issue = [&](int st) {
+    const int kblk = (ka + st * SK) / SK;
+    char *buf = smem + (st % NBUF) * STAGE;
 #pragma unroll
-    for (int j = 0; j < CPT; j++) {
-      const int c = tid + j * DNT;
-      if (c < CHUNKS) {
-        const int p = c / (DBM * CPR), rr = (c / CPR) % DBM, h = c % CPR;
-        rx[j] = *reinterpret_cast<const i4v *>(
-            Xg + ((size_t)p * a.sx + ((size_t)(k0 / SK) * a.ldx + m0 + rr) * SK) * ES + h * 16);
-        if (!ysame)
-          ry[j] = *reinterpret_cast<const i4v *>(
-              Yg + ((size_t)p * a.sy + ((size_t)(k0 / SK) * a.ldy + n0 + rr) * SK) * ES + h * 16);
-      }
-    }
-  };
-  auto stash = [&](char *buf) {
-#pragma unroll
-    for (int j = 0; j < CPT; j++) {
-      const int c = tid + j * DNT;
-      if (c < CHUNKS) {
-        const int p = c / (DBM * CPR), rr = (c / CPR) % DBM, h = c % CPR;
-        *reinterpret_cast<i4v *>(buf + p * PANEL + rr * ROWB + h * 16) = rx[j];
-        if (!ysame) *reinterpret_cast<i4v *>(buf + NL * PANEL + p * PANEL + rr * ROWB + h * 16) = ry[j];
-      }
+    for (int j = 0; j < PERW; j++) {
+      const int i = w + 8 * j;
+      const int o = i / (NL * 4), p = (i / 4) % NL, q = i % 4;
+      const int pos = q * 64 + lane, r = pos >> 1, h = (pos & 1) ^ ((r >> 3) & 1);
+      const char *g = (o == 0)
+          ? Xg + ((size_t)p * a.sx + ((size_t)kblk * a.ldx + m0 + r) * SK) * ES + h * 16
+          : Yg + ((size_t)p * a.sy + ((size_t)kblk * a.ldy + n0 + r) * SK) * ES + h * 16;
+      char *l = buf + o * NL * PANEL + p * PANEL + q * 1024;
+      __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
     }
   };
   const int nst = (kb - ka) / SK;
-  if (nst > 0) {
-    fetch(ka);
-    stash(smem);
+  if (nst > 0) issue(0);
+  if (nst > 1) issue(1);
+  for (int s = 0; s < nst; s++) {
+    if (s + 1 < nst) wait_vm<PERW>();
+    else wait_vm<0>();
     __syncthreads();
-    for (int s = 0; s < nst; s++) {
-      char *cur = smem + (s & 1) * BUF;
-      char *nxt = smem + ((s + 1) & 1) * BUF;
-      const bool more = s + 1 < nst;
-      if (more) fetch(ka + (s + 1) * SK);
-      digit_stage<DT, NL, NL, NL>(cur, ysame ? cur : cur + NL * PANEL, xrow, yrow, lane, acc);
-      if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || !more)) {
+    if (s + 2 < nst) issue(s + 2);
+    const char *cur = smem + (s % NBUF) * STAGE;
+    digit_stage<DT, NL, NL, NL>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
+    if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+      for (int j = 0; j < 2; j++)
 #pragma unroll
-          for (int v = 0; v < 16; v++) {
-            double t = 0.0;
+        for (int v = 0; v < 16; v++) {
+          double t = 0.0;
 #pragma unroll
-            for (int l = NL - 1; l >= 0; l--) t += ldexp((double)acc[l][j][v], -7 * l);
-            facc[j][v] += t;
-          }
+          for (int l = NL - 1; l >= 0; l--) t += ldexp((double)acc[l][j][v], -7 * l);
+          facc[j][v] += t;
+        }
 #pragma unroll
-        for (int l = 0; l < NL; l++)
+      for (int l = 0; l < NL; l++)
 #pragma unroll
-          for (int j = 0; j < 2; j++) acc[l][j] = ACC{};
-      }
-      if (more) stash(nxt);
-      __syncthreads();
+        for (int j = 0; j < 2; j++) acc[l][j] = ACC{};
     }
   }
 
