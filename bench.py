@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large")
-    ap.add_argument("--modes", default="fp64",
+    ap.add_argument("--modes", default="fp64,int8x6l",
                     help="comma list of precisions to run (fp64, int8x5, fp16x3, ...)")
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--maxit", type=int, default=12)
@@ -65,6 +65,20 @@ def calibrate_copy():
     torch.cuda.synchronize()
     print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 8,
                       "bytes_written": b.numel() * 8, "launches": 3}), flush=True)
+
+
+# HBM bytes per launch of the dominant kernel at n=1024, npw=262144, measured
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (separate runs) and scaled by
+# the same counters on a 1 GiB device copy (fetch x2.000, write x1.000).
+TRAFFIC = {
+    "fp64": 12.88e9 + 4.29e9,     # gemm_f64_kernel T*psi: read 12.88 GB, write 4.29 GB
+    "int8x6l": 9.62e9 + 4.29e9,   # digit back-multiply: read 9.62 GB, write 4.29 GB
+}
+
+
+def _slices(prec: str):
+    body = prec.split("x")[1]
+    return int(body.rstrip("l")), body.endswith("l")
 
 
 def run_mode(h, psi, prec, args, n, npw):
@@ -195,14 +209,23 @@ def main():
     head = max(ok.values(), key=lambda r: r["steps_per_s"])
     fp64 = results.get("fp64")
     iters = head["iterations"]
-    flops_bm = 4.0 * n * n * npw
     if head["prec"] == "fp64":
+        # dominant kernel: the T*psi back-multiply, one gemm_f64_kernel launch
+        ops_bm = 4.0 * n * n * npw
         peak, unit, bound = PEAK_FP64_TFLOPS, "TFLOP/s", "mfma"
         kernel = "gemm_f64_kernel (T*psi back-multiply)"
+        alg = f"4*n^2*npw = {ops_bm:.4g} FP64 flop"
     else:
+        # dominant kernel: the digit back-multiply (digit_nt_kernel MODE 1);
+        # the timed call also slices T (n^2 elements, a few microseconds)
+        s_, lean = _slices(head["prec"])
+        nl = s_ - 2 if lean else s_ - 1
+        prods = nl * (nl + 1) // 2
+        ops_bm = prods * 2.0 * n * n * (2 * npw)
         peak, unit, bound = PEAK_INT8_TOPS, "TOP/s", "mfma"
-        kernel = "split back-multiply"
-    achieved = flops_bm / (head["backmul_ms"] * 1e-3) / 1e12
+        kernel = f"digit_nt_kernel back-multiply ({head['prec']}: {prods} int8 digit products)"
+        alg = f"{prods} products x 2*n*n*(2*npw) = {ops_bm:.4g} int8 op"
+    achieved = ops_bm / (head["backmul_ms"] * 1e-3) / 1e12
     line = {
         "metric": "ortho steps/s (S+S->T+T*psi) split-prec vs FP64 at resid<=1e-10; p99 latency",
         "value": head["steps_per_s"],
@@ -226,6 +249,8 @@ def main():
         "iterations": iters,
         "modes": results,
         "fp64_steps_per_s": fp64["steps_per_s"] if fp64 and "error" not in fp64 else None,
+        "speedup_vs_fp64": (head["steps_per_s"] / fp64["steps_per_s"]
+                            if fp64 and "error" not in fp64 else None),
         "roofline": {
             "kernel": kernel,
             "bound": bound,
@@ -233,8 +258,8 @@ def main():
             "peak": peak,
             "unit": unit,
             "frac": achieved / peak,
-            "traffic": None,
-            "algorithmic_per_launch": f"4*n^2*npw = {flops_bm:.4g} flop",
+            "traffic": TRAFFIC.get(head["prec"]) if args.workload == "large" else None,
+            "algorithmic_per_launch": alg,
         },
     }
     if not args.no_cpu_baseline:
