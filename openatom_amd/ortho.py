@@ -13,7 +13,8 @@ library runs every product on the GPU.  Names follow the reference:
   PairCalculator.multiplyPsiV      ckPairCalculator.C:1128-1176
   Ortho.start_calc                 ortho.C:153-266 + do_iteration..collect_error
                                    (:848-971, :109-146)
-  Ortho.array_diag_max             ortho.h:282-313
+  Ortho.diag                       ortho.h:282-313 (the diagonal
+                                   array_diag_max measures against)
   Ortho.acceptSectionLambda        ortho.C:497-608 (lsda: Lambda x 2)
   Ortho.gamma                      ortho.C:565-570 (Gamma = T * Lambda)
 """
