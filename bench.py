@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large")
-    ap.add_argument("--modes", default="fp64,int8x6l",
+    ap.add_argument("--modes", default="fp64,int8x5",
                     help="comma list of precisions to run (fp64, int8x5, fp16x3, ...)")
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--maxit", type=int, default=12)

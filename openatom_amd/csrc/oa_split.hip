@@ -2,19 +2,21 @@
 //
 // Representation.  The K plane-wave doubles are cut into chunks of kchunk
 // columns (a multiple of 32).  Every value x of chunk z is written as
-//     x = 2^(e_z - 7) * sum_{p=1..S} a_p 2^(-7(p-1))  + r,
-// |a_1| <= 127, |a_p| <= 64 (p >= 2), |r| <= 2^(e_z - 8 - 7(S-1)),
-// with one power-of-two exponent e_z per chunk, shared by all rows.  The
-// digits are exact integers; the split is exact in FP64 (scaling by powers
-// of two and subtracting a rounded value lose nothing).
+//     x = 2^(e_z - 7) * sum_{p=1..S} a_p R^(-(p-1))  + r,   R = 254,
+// |a_p| <= 127, |r| <= 2^(e_z - 8) R^(-(S-1)), with one power-of-two exponent
+// e_z per chunk, shared by all rows.  Each digit is the rounded remainder:
+// |u - rint(u)| <= 1/2, so the next u = R (u - rint(u)) stays within 127 and
+// every digit uses the full signed 8-bit range (log2 254 = 7.99 bits per
+// slice).  The digits are exact integers; the recurrence rounds only in the
+// product by R, 2^-53 relative to the remainder, far below 2^-40.
 //
 // Products.  sum_k a_p[i,k] b_q[j,k] over one chunk is accumulated exactly:
 // int8 digits on v_mfma_i32_32x32x32_i8 into int32 (bounded by the chunk
 // length, which the host caps), fp16 / bf16 digits on
-// v_mfma_f32_32x32x16_{f16,bf16} into fp32 promoted to FP64 every 256 k
+// v_mfma_f32_32x32x16_{f16,bf16} into fp32 promoted to FP64 every 128 k
 // (sums stay below 2^24 and are therefore exact).  Products are grouped by
 // level L = p + q <= S + 1 (the Ozaki triangular truncation); level sums are
-// combined in FP64 with weights 2^(-7(L-2)) and the chunk scales.
+// combined in FP64 with weights R^(-(L-2)) and the chunk scales.
 //
 // Layout.  Every digit matrix (rows R, contraction C) is stored k-blocked:
 // element (p, r, c) at p*R*C + ((c / B)*R + r)*B + c % B with B = 32 bytes of
@@ -137,6 +139,22 @@ __device__ __forceinline__ i4v pack_digits(const double *d) {
   return r;
 }
 
+// Digit base R and the level weights R^-l (R^l is exact in FP64 for l <= 12,
+// so each weight is the correctly rounded reciprocal).
+constexpr double RADIX = 254.0;
+__host__ __device__ constexpr double rinv_c(int l) {
+  double p = 1.0;
+  for (int i = 0; i < l; i++) p *= RADIX;
+  return 1.0 / p;
+}
+// Called with indices that are constants after unrolling: the table folds.
+__host__ __device__ __forceinline__ double rinv(int l) {
+  constexpr double t[2 * SMAX] = {rinv_c(0), rinv_c(1), rinv_c(2), rinv_c(3),
+                                  rinv_c(4), rinv_c(5), rinv_c(6), rinv_c(7),
+                                  rinv_c(8), rinv_c(9), rinv_c(10), rinv_c(11)};
+  return t[l];
+}
+
 // Split one value into S digits given its exponent.
 template <int S>
 __device__ __forceinline__ void split_value(double x, int e, double (&d)[S]) {
@@ -145,7 +163,7 @@ __device__ __forceinline__ void split_value(double x, int e, double (&d)[S]) {
   for (int p = 0; p < S; p++) {
     const double a = rint(u);
     d[p] = a;
-    u = (u - a) * 128.0;
+    u = (u - a) * RADIX;
   }
 }
 
@@ -434,9 +452,10 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
   constexpr int NBUF = 3;                     // LDS ring depth
   constexpr int NI = 2 * NL * 4;              // 1 KB wave-loads per stage
   constexpr int PERW = NI / 8;                // per wave (8 waves)
-  // fp32 accumulation is exact only below 2^24 (level sums < 28544 per k for
-  // S <= 5): promote every PROMOTE stages into one FP64 sum per output
-  constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 512 / SK;
+  // fp32 accumulation is exact only below 2^24 (level sums < 6 * 127^2 =
+  // 96774 per k for S <= 6): promote every PROMOTE stages (128 k) into one
+  // FP64 sum per output
+  constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 128 / SK;
   __shared__ __attribute__((aligned(1024))) char smem[NBUF * STAGE];
 
   int bi, bj, z = 0, tile;
@@ -513,7 +532,7 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
         for (int v = 0; v < 16; v++) {
           double t = 0.0;
 #pragma unroll
-          for (int l = NL - 1; l >= 0; l--) t += ldexp((double)acc[l][j][v], -7 * l);
+          for (int l = NL - 1; l >= 0; l--) t += (double)acc[l][j][v] * rinv(l);
           facc[j][v] += t;
         }
 #pragma unroll
@@ -545,7 +564,7 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
                                         (size_t)a.Kp);
           load_digit_quad<DT>(a.DB, off, q4);
 #pragma unroll
-          for (int r = 0; r < 4; r++) pd[4 * g + r] += ldexp(q4[r], -7 * p);
+          for (int r = 0; r < 4; r++) pd[4 * g + r] += q4[r] * rinv(p);
         }
       }
     }
@@ -554,7 +573,7 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
       double val = 0.0;
       if (DT == DT_I8) {
 #pragma unroll
-        for (int l = NL - 1; l >= 0; l--) val += ldexp((double)acc[l][j][v], -7 * l);
+        for (int l = NL - 1; l >= 0; l--) val += (double)acc[l][j][v] * rinv(l);
       } else {
         val = facc[j][v];
       }
@@ -627,8 +646,8 @@ bool decode_prec(int prec, int &dt, int &S) {
   return true;
 }
 
-// Largest level sum of |a_p a_q| per contraction element (|a_1| <= 127,
-// |a_p| <= 64): the int32 chunk length must keep chunk * bound < 2^31.
+// Largest level sum of |a_p a_q| per contraction element (|a_p| <= 127):
+// the int32 chunk length must keep chunk * bound < 2^31.
 long level_bound(int S) {
   long best = 0;
   for (int L = 2; L <= S + 1; L++) {
@@ -636,7 +655,7 @@ long level_bound(int S) {
     for (int p = 1; p < L; p++) {
       const int q = L - p;
       if (q < 1 || p > S || q > S) continue;
-      b += (long)(p == 1 ? 127 : 64) * (q == 1 ? 127 : 64);
+      b += 127L * 127L;
     }
     best = b > best ? b : best;
   }

@@ -2,10 +2,12 @@
 
 The FP64 path is itself pinned to the oracle at 1e-12 (test_gpu_f64.py), so
 it serves as the reference here.  Tolerances follow the digit arithmetic:
-S slices of 7 bits leave a representation error of 2^-(7S+1) of the chunk
-maximum per element, so |dS| / max|S| <~ 2^-(7S-3) (x3: 1e-5, x4: 1e-7,
-x5: 1e-10, x6: 1e-12 with margin).  Digit sums are exact integers, so int8,
-fp16 and bf16 digits must agree to FP64 rounding of the final combination.
+a leading digit of |a_1| <= 127 and S-1 digits in base 254 leave a
+representation error of 2^-8 254^-(S-1) of the chunk maximum per element
+(measured |dS| / max|S|: x3 1.4e-6, x4 2.6e-8, x5 2.9e-11, x6 4.4e-13); the
+bounds below keep a margin of about 10.  Digit sums are exact integers, so
+int8, fp16 and bf16 digits must agree to FP64 rounding of the final
+combination.
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ from openatom_amd import _lib  # noqa: E402
 from openatom_amd.workload import make_states  # noqa: E402
 
 DEV = "cuda"
-TOL = {2: 1e-3, 3: 1e-5, 4: 3e-7, 5: 1e-9, 6: 1e-11}
+TOL = {2: 1e-3, 3: 1e-5, 4: 3e-7, 5: 3e-10, 6: 5e-12}
 
 
 def rel(a, b):
@@ -99,7 +101,7 @@ def test_split_ragged_sizes(handle, n, npw, nkx0):
     assert rel(handle.backmul(M, psi, prec="int8x5"), handle.backmul(M, psi, prec="fp64")) < 1e-6
 
 
-@pytest.mark.parametrize("prec", ["int8x6l", "int8x6"])
+@pytest.mark.parametrize("prec", ["int8x5", "fp16x5", "int8x6l"])
 def test_split_step_meets_residual_target(handle, prec):
     """n=512, npw=65536 (the plain-vs-split configuration): one split step
     leaves the orthonormality residual <= 1e-10 and T within 1e-10 of FP64."""
