@@ -173,6 +173,69 @@ int oa_ortho_step(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
                   double *S, double *T, double *psi_out, long ldout,
                   int *iters, double *err);
 
+/* ---- batched independent instances (grouped launches) -----------------
+ *
+ * Many independent orthonormalisation problems of uneven size -- path-integral
+ * beads, k-points, tempering replicas, each its own GSP/PC/Ortho instance
+ * set in the reference (cpaimd.C:600-664 loops over spin, temper, k-point
+ * and bead; cpaimd.C:2566-2593 builds each instance's PairCalculator pair
+ * and Ortho array) -- advance together: every phase
+ * of the step (chunk maxima, slicing, overlap products, reduction, each S->T
+ * iteration, back-multiply) is ONE persistent grouped launch over all
+ * instances, walking a device table of per-instance arguments.  Results are
+ * those of oa_ortho_step on each instance alone, bit for bit (the digit
+ * products are exact and the chunking depends on (n, npw) only). */
+typedef struct oa_instance {
+  int n;              /* states                                          */
+  int npw;            /* plane waves (2*npw doubles per state)           */
+  int nkx0;           /* leading kx=0 points (weight 1/2 in S)           */
+  int nspin;          /* 1: B = S/2; 2 (lsda): B = S                     */
+  const double *psi;  /* device [n][ldpsi], read                          */
+  long ldpsi;
+  double *psi_out;    /* device [n][ldout], written (T psi)              */
+  long ldout;
+  double *S;          /* device n*n or NULL (kept in the batch workspace) */
+  double *T;          /* device n*n or NULL (kept in the batch workspace) */
+  double tol;         /* invsqr_tolerance                                */
+  int maxit;          /* invsqr_max_iter                                 */
+  int pad;
+} oa_instance;
+
+typedef struct oa_batch *oa_batch_t;
+
+/* A batch lives on its handle's device and orders its work after the
+ * handle's stream. */
+int oa_batch_create(oa_handle_t h, oa_batch_t *b);
+int oa_batch_destroy(oa_batch_t b);
+/* (Re)plan for `count` instances at precision `prec` (OA_PREC_FP64 or a
+ * slice mode).  Synchronous: waits for the batch's previous work, grows the
+ * device workspace when needed and uploads the argument tables.  Tile
+ * choices come from the batch's tuning table (oa_batch_tune), else from a
+ * size rule. */
+int oa_batch_plan(oa_batch_t b, int count, const oa_instance *inst, int prec);
+/* Bytes of device memory the planned batch holds (workspace + tables). */
+size_t oa_batch_footprint(oa_batch_t b);
+/* Device bytes one instance of (n, npw) needs at `prec` (admission). */
+size_t oa_instance_footprint(int n, int npw, int prec);
+/* Enqueue one step of every planned instance on the handle's stream: no
+ * allocation, no host synchronisation (capturable). */
+int oa_batch_run(oa_batch_t b);
+/* The same step as a hipGraph replay: the first call after a plan captures
+ * oa_batch_run into a graph; later calls replay it. */
+int oa_batch_launch(oa_batch_t b);
+/* Synchronous copy of the per-instance S->T reports (count entries). */
+int oa_batch_reports(oa_batch_t b, oa_ns_report *host, int count);
+/* Project autotuner: for each size bucket (n rounded up to 32) pick the
+ * digit-GEMM tile (128 or 64) and the S->T GEMM tile (WM 4 or 2) by timing
+ * the whole planned batch `reps` times per candidate (coordinate descent
+ * over buckets), keep the choices in the batch's table and re-plan. */
+int oa_batch_tune(oa_batch_t b, int reps);
+/* Tile choices of planned instance i (digit tile, S->T GEMM WM). */
+int oa_batch_config(oa_batch_t b, int i, int *tile, int *wm);
+/* Tuning table as text ("bucket tile wm" per line) and back. */
+int oa_batch_get_table(oa_batch_t b, char *buf, int bytes);
+int oa_batch_set_table(oa_batch_t b, const char *text);
+
 #ifdef __cplusplus
 }
 #endif

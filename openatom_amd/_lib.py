@@ -62,6 +62,14 @@ class oa_ns_report(ctypes.Structure):
                 ("error", ctypes.c_double)]
 
 
+class oa_instance(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("npw", ctypes.c_int), ("nkx0", ctypes.c_int),
+                ("nspin", ctypes.c_int), ("psi", ctypes.c_void_p), ("ldpsi", ctypes.c_long),
+                ("psi_out", ctypes.c_void_p), ("ldout", ctypes.c_long),
+                ("S", ctypes.c_void_p), ("T", ctypes.c_void_p), ("tol", ctypes.c_double),
+                ("maxit", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -101,6 +109,18 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_backmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
     _sig(L.oa_ortho_step, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp, _l,
          ctypes.POINTER(_i), ctypes.POINTER(_d))
+    _sig(L.oa_batch_create, _i, _vp, ctypes.POINTER(_vp))
+    _sig(L.oa_batch_destroy, _i, _vp)
+    _sig(L.oa_batch_plan, _i, _vp, _i, ctypes.POINTER(oa_instance), _i)
+    _sig(L.oa_batch_footprint, ctypes.c_size_t, _vp)
+    _sig(L.oa_instance_footprint, ctypes.c_size_t, _i, _i, _i)
+    _sig(L.oa_batch_run, _i, _vp)
+    _sig(L.oa_batch_launch, _i, _vp)
+    _sig(L.oa_batch_reports, _i, _vp, ctypes.POINTER(oa_ns_report), _i)
+    _sig(L.oa_batch_tune, _i, _vp, _i)
+    _sig(L.oa_batch_config, _i, _vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i))
+    _sig(L.oa_batch_get_table, _i, _vp, ctypes.c_char_p, _i)
+    _sig(L.oa_batch_set_table, _i, _vp, ctypes.c_char_p)
     _lib = L
     return L
 

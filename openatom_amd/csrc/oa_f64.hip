@@ -205,15 +205,11 @@ __device__ __forceinline__ ProbRegs pick_problem(const oa_gemm_problem &p0,
   return r;
 }
 
+// One (BM x BM) output tile `logical` of C = alpha*op(X)*op(Y) + beta*C + d*I.
 template <bool TX, bool TY, int WM>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void gemm_f64_kernel(oa_gemm_problem p0, oa_gemm_problem p1, const int32_t *skip) {
-  if (skip && *skip) return;
-  const ProbRegs P = pick_problem(p0, p1);
+__device__ __forceinline__ void gemm_body(const ProbRegs &P, int logical, double *smem) {
   constexpr int BM = 32 * WM;
-  __shared__ __attribute__((aligned(16))) double smem[4 * BM * LS];
   const int tmn = cdiv(P.M, BM), tnn = cdiv(P.N, BM);
-  const int logical = xcd_logical(blockIdx.x, gridDim.x);
   if (logical >= tmn * tnn) return;
   const int tm = logical % tmn, tn = logical / tmn;
   const int m0 = tm * BM, n0 = tn * BM;
@@ -260,6 +256,61 @@ void gemm_f64_kernel(oa_gemm_problem p0, oa_gemm_problem p1, const int32_t *skip
         }
       }
   }
+}
+
+template <bool TX, bool TY, int WM>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void gemm_f64_kernel(oa_gemm_problem p0, oa_gemm_problem p1, const int32_t *skip) {
+  if (skip && *skip) return;
+  const ProbRegs P = pick_problem(p0, p1);
+  constexpr int BM = 32 * WM;
+  __shared__ __attribute__((aligned(16))) double smem[4 * BM * LS];
+  gemm_body<TX, TY, WM>(P, xcd_logical(blockIdx.x, gridDim.x), smem);
+}
+
+// ------------------------------------------------------------------ grouped
+// Instance owning item t: the largest i with prefix[i] <= t.
+__device__ __forceinline__ int group_find(const int *__restrict__ prefix, int count, int t) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Workgroups of XCD x (= b % 8) walk the contiguous item range
+// [x*per, (x+1)*per); the grid is a multiple of 8.
+template <class F>
+__device__ __forceinline__ void for_items(int total, F &&f) {
+  const int per = cdiv(total, 8), slots = (int)gridDim.x >> 3, x = blockIdx.x & 7;
+  for (int pos = blockIdx.x >> 3; pos < per; pos += slots) {
+    const int item = x * per + pos;
+    if (item < total) f(item);
+  }
+}
+
+// Many row-major NN problems in one persistent launch; items are output
+// tiles; a problem whose skip flag is set contributes no work.
+template <int WM>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void gemm_f64_grouped(const oa_gemm_item *__restrict__ t, const int *__restrict__ prefix,
+                      int count) {
+  constexpr int BM = 32 * WM;
+  __shared__ __attribute__((aligned(16))) double smem[4 * BM * LS];
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    const oa_gemm_item &it = t[i];
+    if (it.skip && *it.skip) return;
+    ProbRegs P;
+    P.M = it.p.M; P.N = it.p.N; P.K = it.p.K;
+    P.X = it.p.X; P.ldx = it.p.ldx; P.Y = it.p.Y; P.ldy = it.p.ldy;
+    P.C = it.p.C; P.ldc = it.p.ldc;
+    P.alpha = it.p.alpha; P.beta = it.p.beta; P.diag_add = it.p.diag_add;
+    gemm_body<false, false, WM>(P, item - prefix[i], smem);
+    __syncthreads();
+  });
 }
 
 // Split-K partial products over the plane-wave index.
@@ -454,6 +505,115 @@ __global__ void ns_finish_kernel(const double *__restrict__ A0, const double *__
     T[idx] = src[idx];
 }
 
+// Grouped S->T bookkeeping over many instances (oa_ns_args tables).
+//   INIT   items of NT elements: B = S/2 or S, A = I, report cleared.
+//   RESID  item w < nwg: partial[w] = sum of (Aold - Anew)^2 over elements
+//          w*NT + t + j*nwg*NT (a fixed order for a given nwg).
+//   UPDATE one item per instance: Ortho::collect_error (ortho.C:109-146).
+//   FINISH items of NT elements: T = A[iterations % 2].
+__global__ __launch_bounds__(NT) void ns_init_grouped(const oa_ns_args *__restrict__ t,
+                                                      const int *__restrict__ prefix,
+                                                      int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    const oa_ns_args &a = t[i];
+    const int w = item - prefix[i];
+    const long nn = (long)a.n * a.n, idx = (long)w * NT + threadIdx.x;
+    if (idx < nn) {
+      const int r = (int)(idx / a.n), c = (int)(idx % a.n);
+      a.B[0][idx] = (a.nspin == 1) ? a.S[idx] / 2.0 : a.S[idx];
+      a.A[0][idx] = (r == c) ? 1.0 : 0.0;
+    }
+    if (w == 0 && threadIdx.x == 0) {
+      a.rep->iterations = 0;
+      a.rep->done = 0;
+      a.rep->status = OA_SUCCESS;
+      a.rep->pad = 0;
+      a.rep->error = 0.0;
+    }
+  });
+}
+
+template <int PAR>
+__global__ __launch_bounds__(NT) void ns_resid_grouped(const oa_ns_args *__restrict__ t,
+                                                       const int *__restrict__ prefix,
+                                                       int count) {
+  __shared__ double red[NT];
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    const oa_ns_args &a = t[i];
+    if (a.rep->done) return;
+    const int w = item - prefix[i];
+    const long nn = (long)a.n * a.n;
+    const double *Aold = a.A[PAR], *Anew = a.A[PAR ^ 1];
+    double s = 0.0;
+    for (long idx = (long)w * NT + threadIdx.x; idx < nn; idx += (long)a.nwg * NT) {
+      const double d = Aold[idx] - Anew[idx];
+      s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[w] = red[0];
+    __syncthreads();
+  });
+}
+
+__global__ __launch_bounds__(NT) void ns_update_grouped(const oa_ns_args *__restrict__ t,
+                                                        const int *__restrict__ prefix,
+                                                        int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    const oa_ns_args &a = t[i];
+    if (threadIdx.x != 0 || a.rep->done) return;
+    double s = 0.0;
+    for (int w = 0; w < a.nwg; w++) s += a.partial[w];
+    const double err = sqrt(s / ((double)a.n * (double)a.n));
+    const int it = a.rep->iterations + 1;
+    a.rep->iterations = it;
+    a.rep->error = err;
+    if (!(err > a.tol && it < a.maxit)) {
+      a.rep->done = 1;
+      a.rep->status = (it >= a.maxit) ? OA_ERR_NOT_CONVERGED : OA_SUCCESS;
+    }
+  });
+}
+
+__global__ __launch_bounds__(NT) void ns_finish_grouped(const oa_ns_args *__restrict__ t,
+                                                        const int *__restrict__ prefix,
+                                                        int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    const oa_ns_args &a = t[i];
+    const long nn = (long)a.n * a.n, idx = (long)(item - prefix[i]) * NT + threadIdx.x;
+    const double *src = (a.rep->iterations & 1) ? a.A[1] : a.A[0];
+    if (idx < nn) a.T[idx] = src[idx];
+  });
+}
+
+template <class K>
+int grouped_grid(K kernel, int total) {
+  int per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, NT, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  static thread_local int cached_dev = -1, cached_cus = 256;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev != cached_dev) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0)
+      cached_cus = cus;
+    cached_dev = dev;
+  }
+  const int cap = cdiv(cached_cus * per_cu, 8) * 8;
+  const int g = cdiv(max(total, 1), 8) * 8;
+  return g < cap ? g : cap;
+}
+
 template <bool TX, bool TY>
 int launch_gemm(hipStream_t s, const oa_gemm_problem *p, int nprob, const int32_t *skip) {
   int tiles128 = 0;
@@ -639,4 +799,48 @@ int oa_launch_s_to_t(hipStream_t s, int n, const double *S, int nspin, double to
     st = oa_ns_iterate(s, n, it, tol, maxit, rep, ws);
   if (st) return st;
   return oa_ns_finish(s, n, T, rep, ws);
+}
+
+int oa_gemm_grouped(hipStream_t s, int wm, const oa_group &g) {
+  if (g.count <= 0 || g.total <= 0) return OA_SUCCESS;
+  const oa_gemm_item *t = reinterpret_cast<const oa_gemm_item *>(g.args);
+  if (wm == 4)
+    hipLaunchKernelGGL(gemm_f64_grouped<4>, dim3(grouped_grid(gemm_f64_grouped<4>, g.total)),
+                       dim3(NT), 0, s, t, g.prefix, g.count);
+  else if (wm == 2)
+    hipLaunchKernelGGL(gemm_f64_grouped<2>, dim3(grouped_grid(gemm_f64_grouped<2>, g.total)),
+                       dim3(NT), 0, s, t, g.prefix, g.count);
+  else
+    return OA_ERR_UNSUPPORTED;
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+int oa_ns_items(int phase, int n, int nwg) {
+  const long nn = (long)n * n;
+  switch (phase) {
+    case OA_NS_INIT:
+    case OA_NS_FINISH: return (int)((nn + NT - 1) / NT);
+    case OA_NS_RESID0:
+    case OA_NS_RESID1: return nwg;
+    case OA_NS_UPDATE: return 1;
+  }
+  return 0;
+}
+
+int oa_ns_grouped(hipStream_t s, int phase, const oa_group &g) {
+  if (g.count <= 0 || g.total <= 0) return OA_SUCCESS;
+  const oa_ns_args *t = reinterpret_cast<const oa_ns_args *>(g.args);
+#define OA_NSG(K)                                                                      \
+  hipLaunchKernelGGL(K, dim3(grouped_grid(K, g.total)), dim3(NT), 0, s, t, g.prefix, \
+                     g.count)
+  switch (phase) {
+    case OA_NS_INIT: OA_NSG(ns_init_grouped); break;
+    case OA_NS_RESID0: OA_NSG(ns_resid_grouped<0>); break;
+    case OA_NS_RESID1: OA_NSG(ns_resid_grouped<1>); break;
+    case OA_NS_UPDATE: OA_NSG(ns_update_grouped); break;
+    case OA_NS_FINISH: OA_NSG(ns_finish_grouped); break;
+    default: return OA_ERR_UNSUPPORTED;
+  }
+#undef OA_NSG
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }

@@ -97,3 +97,129 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K,
                             const double *M, int transM, const double *psi,
                             long ldpsi, int kx0_end, double alpha, double beta,
                             double *out, long ldout, void *ws, size_t ws_bytes);
+
+// ---------------------------------------------------------------- grouped
+// A grouped launch walks `total` work items spread over `count` instances:
+// item t belongs to instance i with prefix[i] <= t < prefix[i + 1], and
+// args[i] holds that instance's arguments.  Both tables live in device
+// memory; the launch is persistent (grid sized to the device, each workgroup
+// loops over items) and maps items to XCDs in contiguous ranges.
+struct oa_group {
+  const void *args;   // device array of per-instance argument structs
+  const int *prefix;  // device int[count + 1]
+  int count;
+  int total;          // prefix[count]
+};
+
+// Per-instance arguments of the grouped kernels (plain data, uploaded as
+// arrays).  Field meanings follow the single-instance kernels.
+struct oa_chunkmax_args {
+  const double *psi;
+  long ld;
+  int n, K, kx0_end, kchunk, nz, pad;
+  unsigned long long *maxS, *maxB;
+};
+struct oa_slice_args {
+  const double *psi;
+  long ld;
+  int n, K, kx0_end, kchunk;
+  const unsigned long long *maxS, *maxB;
+  void *DS, *DB;
+  int np, Kp, want_ds, vec;
+};
+struct oa_digit_args {
+  const void *X;  // k-blocked digits, rows m
+  long ldx, sx;   // rows of the layout and slice stride (elements)
+  const void *Y;  // k-blocked digits, rows n
+  long ldy, sy;
+  int M, N;       // output extent
+  int kc;         // contraction length per split (elements) or total
+  int kfull;      // contraction extent of the digit arrays (padded)
+  int ntiles, nz; // MODE 0: symmetric tiles, splits
+  double *part;   // MODE 0 output
+  const unsigned long long *chunk_max;  // MODE 0: per chunk scale source
+  double *out;    // MODE 1 output
+  long ldo;
+  double alpha, beta;
+  const int *fexp;                   // per output row exponent (T rows)
+  const double *diagv;               // FP64 diagonal of T
+  const void *DB;                    // psi digits (rows Kp, contraction np)
+  long dbs;                          // DB slice stride (elements)
+  int np, Kp;
+  const unsigned long long *colmax;  // per chunk of output columns
+  int kchunk, pad;
+};
+struct oa_sreduce_args {
+  const double *part;
+  int n, nz;
+  double alpha;
+  const double *psi;
+  long ld;
+  int kx0_end, pad;
+  double *S;
+  double diag;
+  unsigned long long *maxdev;
+};
+struct oa_rows_args {
+  const double *M;
+  int n, trans;
+  void *D;
+  int npk, pad;
+  int *fexp;
+  double *diagv;
+};
+
+// Split-precision layout of one instance (digit workspace offsets).
+struct oa_split_plan {
+  int dt, S, es, tile;
+  int np, Kp;       // padded rows / columns (multiples of the tile)
+  int kchunk, nz;   // psi chunks: scale blocks and split-K units
+  int ntiles;       // upper-triangle tiles of S
+  size_t off_ds, off_db, off_dt, off_maxS, off_maxB, off_fexp, off_diag, off_part, off_md;
+  size_t total;
+};
+oa_split_plan oa_split_plan_for(int prec, int n, int K, int tile);
+// Fill the per-phase argument structs of one instance whose digit workspace
+// starts at `ws` (layout `pl`).  maxS/maxB may point outside ws (batched
+// counters); null selects the plan's own slots.
+struct oa_split_inst_args {
+  oa_chunkmax_args cm;
+  oa_slice_args sl;
+  oa_digit_args d0;
+  oa_sreduce_args rd;
+  oa_rows_args rows;
+  oa_digit_args d1;
+  int items_cm, items_sl, items_d0, items_rd, items_rows, items_d1;
+};
+void oa_split_inst_fill(const oa_split_plan &pl, int prec, int n, int K, const double *psi,
+                        long ld, int kx0_end, double alpha, double *S, const double *T,
+                        int transT, double beta, double *out, long ldout, char *ws,
+                        unsigned long long *maxS, unsigned long long *maxB,
+                        oa_split_inst_args *a);
+enum oa_split_phase {
+  OA_SP_CHUNKMAX = 0, OA_SP_SLICE, OA_SP_DIGIT0, OA_SP_REDUCE, OA_SP_ROWS, OA_SP_DIGIT1
+};
+// One grouped launch of a split phase for instances sharing `prec` and `tile`.
+int oa_split_grouped(hipStream_t s, int prec, int tile, int phase, const oa_group &g);
+int oa_fill_u64(hipStream_t s, unsigned long long *p, long n);
+
+// FP64 grouped kernels: GEMM (row-major NN, per-problem skip flag) and the
+// S->T bookkeeping kernels.
+struct oa_gemm_item {
+  oa_gemm_problem p;
+  const int32_t *skip;  // may be null
+};
+struct oa_ns_args {
+  const double *S;
+  int n, nspin;
+  double *A[2], *B[2], *C;
+  double *partial;
+  int nwg, maxit;
+  double tol;
+  double *T;
+  oa_ns_report *rep;
+};
+enum oa_ns_phase { OA_NS_INIT = 0, OA_NS_RESID0, OA_NS_RESID1, OA_NS_UPDATE, OA_NS_FINISH };
+int oa_gemm_grouped(hipStream_t s, int wm, const oa_group &g);
+int oa_ns_grouped(hipStream_t s, int phase, const oa_group &g);
+int oa_ns_items(int phase, int n, int nwg);

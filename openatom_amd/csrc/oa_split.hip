@@ -16,28 +16,34 @@
 // v_mfma_f32_32x32x16_{f16,bf16} into fp32 promoted to FP64 every 128 k
 // (sums stay below 2^24 and are therefore exact).  Products are grouped by
 // level L = p + q <= S + 1 (the Ozaki triangular truncation); level sums are
-// combined in FP64 with weights R^(-(L-2)) and the chunk scales.
+// combined in FP64 with weights R^(-(L-2)) and the chunk scales.  Because the
+// level sums are exact, every result is independent of the tile shape, the
+// split-K grouping and the launch (single or grouped): only the chunk length
+// enters the numerics, and it depends on (n, K) alone.
 //
 // Layout.  Every digit matrix (rows R, contraction C) is stored k-blocked:
 // element (p, r, c) at p*R*C + ((c / B)*R + r)*B + c % B with B = 32 bytes of
-// digits, so one pipeline stage of a workgroup (128 rows x 32 bytes) is one
-// contiguous 4 KB block per slice.
+// digits, so one pipeline stage of a workgroup (TILE rows x 32 bytes) is one
+// contiguous block per slice.
 //
-// Kernels:
-//   chunk_max_kernel   max |psi| per chunk (two variants: without / with the
-//                      leading kx0 columns), order-free via atomicMax on the
-//                      IEEE bits of non-negative doubles.
-//   slice_psi_kernel   psi -> digits in two layouts: DS[p][row][k] (rows
-//                      contiguous in k, for S = psi psi^T; kx0 columns zeroed,
-//                      their term is added in FP64) and DB[p][k][row] (rows
-//                      contiguous in the state index, for T psi), the second
-//                      through an LDS transpose.
-//   slice_rows_kernel  an n x n matrix -> digits with one exponent per row.
-//   digit_nt_kernel    C[m][n] = sum_k X[m][k] Y[n][k] over digit slices:
-//                      MODE 0 = split-K partial tiles of S (upper triangle),
-//                      MODE 1 = back-multiply epilogue out = a*T*psi + b*out.
-//   split_reduce_kernel  fixed-order sum of the partial tiles, FP64 G=0 term,
-//                      mirror, max|S - d I|.
+// Kernels (each a body over work items, launched either for one instance
+// with arguments by value or grouped over many instances from device tables):
+//   chunk_max     max |psi| per chunk (two variants: without / with the
+//                 leading kx0 columns), order-free via atomicMax on the IEEE
+//                 bits of non-negative doubles.
+//   slice_psi     psi -> digits in two layouts: DS[p][row][k] (rows
+//                 contiguous in k, for S = psi psi^T; kx0 columns zeroed,
+//                 their term is added in FP64) and DB[p][k][row] (rows
+//                 contiguous in the state index, for T psi), the second
+//                 through an LDS transpose.
+//   slice_rows    an n x n matrix -> digits with one exponent per row.
+//   digit_nt      C[m][n] = sum_k X[m][k] Y[n][k] over digit slices:
+//                 MODE 0 = split-K partial tiles of S (upper triangle),
+//                 MODE 1 = back-multiply epilogue out = a*T*psi + b*out.
+//                 TILE 128: 512 threads, 8 waves as 4 x 2, wave tile 32 x 64;
+//                 TILE 64: 256 threads, 4 waves as 2 x 2, wave tile 32 x 32.
+//   split_reduce  fixed-order sum of the partial tiles, FP64 G=0 term,
+//                 mirror, max|S - d I|.
 #include <math.h>
 
 #include "oa_internal.h"
@@ -50,13 +56,25 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 
-constexpr int DNT = 512;  // threads per digit-GEMM workgroup (8 waves, 4 x 2)
-constexpr int DBM = 128;  // workgroup tile
 constexpr int DBK = 32;   // chunk granularity (elements); a stage holds 32 bytes per row
 constexpr int SMAX = 6;
+constexpr int SLT = 64;   // slicing tile (rows x columns)
 constexpr long KCHUNK_TARGET = 8192;  // columns per scale block / split-K chunk
 
 __host__ __device__ inline int cdv(int a, int b) { return (a + b - 1) / b; }
+
+template <int TILE>
+struct TileCfg;
+template <>
+struct TileCfg<128> {
+  static constexpr int NW = 8;  // waves: 4 rows x 2 columns
+  static constexpr int WJ = 2;  // 32-column MFMA blocks per wave
+};
+template <>
+struct TileCfg<64> {
+  static constexpr int NW = 4;  // waves: 2 rows x 2 columns
+  static constexpr int WJ = 1;
+};
 
 // Digit element type: int8 or 16-bit float.  ES = bytes per element.
 enum DType { DT_I8 = 0, DT_F16 = 1, DT_BF16 = 2 };
@@ -67,21 +85,18 @@ template <>
 struct DTraits<DT_I8> {
   typedef int8_t elem;
   static constexpr int ES = 1;
-  static constexpr int KI = 32;  // contraction per MFMA
   typedef i16v acc_t;
 };
 template <>
 struct DTraits<DT_F16> {
   typedef _Float16 elem;
   static constexpr int ES = 2;
-  static constexpr int KI = 16;
   typedef f16v acc_t;
 };
 template <>
 struct DTraits<DT_BF16> {
   typedef __bf16 elem;
   static constexpr int ES = 2;
-  static constexpr int KI = 16;
   typedef f16v acc_t;
 };
 
@@ -115,28 +130,8 @@ __device__ __forceinline__ int chunk_exp(double m) {
   return (f <= 127.0 / 128.0) ? E : E + 1;
 }
 
-// Pack 16 bytes of digits (16 int8 or 8 16-bit floats starting at d[0]).
-template <int DT>
-__device__ __forceinline__ i4v pack_digits(const double *d) {
-  i4v r;
-  if constexpr (DT == DT_I8) {
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-      unsigned u = 0;
-#pragma unroll
-      for (int b = 0; b < 4; b++) u |= ((unsigned)(int)d[4 * w + b] & 0xffu) << (8 * b);
-      r[w] = (int)u;
-    }
-  } else {
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-      const typename DTraits<DT>::elem lo = to_digit<DT>(d[2 * w]), hi = to_digit<DT>(d[2 * w + 1]);
-      const unsigned u = (unsigned)__builtin_bit_cast(unsigned short, lo) |
-                         ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
-      r[w] = (int)u;
-    }
-  }
-  return r;
+__device__ __forceinline__ double chunk_scale(const unsigned long long *mx, int z) {
+  return ldexp(1.0, 7 - chunk_exp(__longlong_as_double((long long)mx[z])));
 }
 
 // Digit base R and the level weights R^-l (R^l is exact in FP64 for l <= 12,
@@ -167,27 +162,57 @@ __device__ __forceinline__ void split_value(double x, int e, double (&d)[S]) {
   }
 }
 
-__global__ void fill_u64(unsigned long long *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0ull;
+// ------------------------------------------------------------ work items
+// Instance owning item t: the largest i with prefix[i] <= t.
+__device__ __forceinline__ int group_find(const int *__restrict__ prefix, int count, int t) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
 }
 
-// grid (nz, row groups of 8): max over rows x chunk columns; maxS skips the
+// The dispatcher places workgroup b on XCD b % 8.  Workgroups of XCD x walk
+// the contiguous item range [x*per, (x+1)*per), so items that share operand
+// panels (neighbouring tiles of one instance) share one L2.  The grid is a
+// multiple of 8; with grid = 8*per every workgroup takes one item.
+template <class F>
+__device__ __forceinline__ void for_items(int total, F &&f) {
+  const int per = cdv(total, 8), slots = (int)gridDim.x >> 3, x = blockIdx.x & 7;
+  for (int pos = blockIdx.x >> 3; pos < per; pos += slots) {
+    const int item = x * per + pos;
+    if (item < total) f(item);
+  }
+}
+
+// Single-instance launches use one workgroup per item (grid = 8*per), so no
+// loop is needed: workgroup b takes item (b % 8)*per + b/8.
+__device__ __forceinline__ int single_item(int total) {
+  const int per = cdv(total, 8);
+  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+__global__ void fill_u64(unsigned long long *p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// ------------------------------------------------------------ chunk max
+// Item (z, row group of 8): max over rows x chunk columns; maxS skips the
 // leading kx0 columns, maxB includes them.
-__global__ __launch_bounds__(256) void chunk_max_kernel(const double *__restrict__ psi, long ld,
-                                                        int n, int K, int kx0_end, int kchunk,
-                                                        unsigned long long *maxS,
-                                                        unsigned long long *maxB) {
-  const int z = blockIdx.x;
-  const int k0 = z * kchunk, k1 = min(K, k0 + kchunk);
+__device__ __forceinline__ void chunk_max_body(const oa_chunkmax_args &a, int item) {
+  const int z = item % a.nz, r0 = (item / a.nz) * 8;
+  const int k0 = z * a.kchunk, k1 = min(a.K, k0 + a.kchunk);
   double ms = 0.0, mb = 0.0;
-  const int r0 = blockIdx.y * 8;
-  for (int r = r0; r < min(n, r0 + 8); r++) {
-    const double *row = psi + (long)r * ld;
+  for (int r = r0; r < min(a.n, r0 + 8); r++) {
+    const double *row = a.psi + (long)r * a.ld;
     for (int k = k0 + threadIdx.x; k < k1; k += 256) {
       const double v = fabs(row[k]);
       mb = fmax(mb, v);
-      if (k >= kx0_end) ms = fmax(ms, v);
+      if (k >= a.kx0_end) ms = fmax(ms, v);
     }
   }
   __shared__ double rs[256], rb[256];
@@ -202,21 +227,27 @@ __global__ __launch_bounds__(256) void chunk_max_kernel(const double *__restrict
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicMax(maxS + z, (unsigned long long)__double_as_longlong(rs[0]));
-    atomicMax(maxB + z, (unsigned long long)__double_as_longlong(rb[0]));
+    atomicMax(a.maxS + z, (unsigned long long)__double_as_longlong(rs[0]));
+    atomicMax(a.maxB + z, (unsigned long long)__double_as_longlong(rb[0]));
   }
+  __syncthreads();
 }
 
-// Per-chunk scale factors 2^(7 - e) for both digit layouts.
-__global__ void chunk_scale_kernel(const unsigned long long *__restrict__ maxS,
-                                   const unsigned long long *__restrict__ maxB, int nz,
-                                   double *scS, double *scB) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= nz) return;
-  scS[z] = ldexp(1.0, 7 - chunk_exp(__longlong_as_double((long long)maxS[z])));
-  scB[z] = ldexp(1.0, 7 - chunk_exp(__longlong_as_double((long long)maxB[z])));
+__global__ __launch_bounds__(256) void chunk_max_kernel(oa_chunkmax_args a, int total) {
+  const int item = single_item(total);
+  if (item < total) chunk_max_body(a, item);
 }
 
+__global__ __launch_bounds__(256) void chunk_max_grouped(const oa_chunkmax_args *__restrict__ t,
+                                                         const int *__restrict__ prefix,
+                                                         int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    chunk_max_body(t[i], item - prefix[i]);
+  });
+}
+
+// ------------------------------------------------------------ slicing
 // k-blocked element offset (elements) of (row r, contraction c) in one slice.
 template <int DT>
 __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
@@ -224,47 +255,47 @@ __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
   return ((size_t)(c / B) * rows + r) * B + (c % B);
 }
 
-// grid (Kp/64, np/64), 256 threads; tile 64 rows x 64 columns.  Thread t owns
-// columns 2(t%32), 2(t%32)+1 of rows t/32 + 8q (q < 8): coalesced 16-byte
-// reads.  DS[p] (rows np, contraction Kp) gets the S-layout digits (kx0
-// columns zeroed, their term is exact FP64); DB[p] (rows Kp, contraction np)
-// gets all columns through an LDS transpose.  want_ds == 0 skips DS.
+// Item = 64 rows x 64 columns (item % (Kp/64) selects the columns).  Thread t
+// owns columns 2(t%32), 2(t%32)+1 of rows t/32 + 8q (q < 8): coalesced
+// 16-byte reads.  DS[p] (rows np, contraction Kp) gets the S-layout digits
+// (kx0 columns zeroed, their term is exact FP64); DB[p] (rows Kp, contraction
+// np) gets all columns through an LDS transpose.  want_ds == 0 skips DS.
 template <int DT, int S>
-__global__ __launch_bounds__(256) void slice_psi_kernel(
-    const double *__restrict__ psi, long ld, int n, int K, int kx0_end, int kchunk,
-    const double *__restrict__ scS, const double *__restrict__ scB,
-    typename DTraits<DT>::elem *DS, typename DTraits<DT>::elem *DB, int np, int Kp,
-    int want_ds, int vec) {
+__device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
   typedef typename DTraits<DT>::elem E;
   constexpr int ES = DTraits<DT>::ES;
-  constexpr int PITCH = 64 + 16 / ES;  // LDS row pitch in elements (16-byte aligned rows)
+  constexpr int PITCH = SLT + 16 / ES;  // LDS row pitch in elements (16-byte aligned rows)
   constexpr int PER = 16 / ES;
-  __shared__ __attribute__((aligned(16))) E lt[S][64 * PITCH];
+  __shared__ __attribute__((aligned(16))) E lt[S][SLT * PITCH];
+  E *DS = reinterpret_cast<E *>(a.DS);
+  E *DB = reinterpret_cast<E *>(a.DB);
+  const int nkt = a.Kp / SLT;
   const int t = threadIdx.x;
-  const int i0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+  const int i0 = (item / nkt) * SLT, k0 = (item % nkt) * SLT;
   const int cp = t % 32, rg = t / 32;
   const int kc = k0 + 2 * cp;  // first of this thread's two columns
-  const int z = min(kc, K - 1 > 0 ? K - 1 : 0) / kchunk;
-  const double sS = scS[z], sB = scB[z];
-  const size_t sds = (size_t)np * Kp, sdb = (size_t)Kp * np;
+  const int z = min(kc, a.K - 1 > 0 ? a.K - 1 : 0) / a.kchunk;
+  const double sS = chunk_scale(a.maxS, z), sB = chunk_scale(a.maxB, z);
+  const size_t sds = (size_t)a.np * a.Kp, sdb = (size_t)a.Kp * a.np;
 #pragma unroll 2
   for (int q = 0; q < 8; q++) {
     const int ii = rg + 8 * q, i = i0 + ii;
     double x[2];
-    if (vec && i < n && kc + 1 < K) {
-      const double2 v = *reinterpret_cast<const double2 *>(psi + (long)i * ld + kc);
+    if (a.vec && i < a.n && kc + 1 < a.K) {
+      const double2 v = *reinterpret_cast<const double2 *>(a.psi + (long)i * a.ld + kc);
       x[0] = v.x;
       x[1] = v.y;
     } else {
 #pragma unroll
-      for (int e = 0; e < 2; e++) x[e] = (i < n && kc + e < K) ? psi[(long)i * ld + kc + e] : 0.0;
+      for (int e = 0; e < 2; e++)
+        x[e] = (i < a.n && kc + e < a.K) ? a.psi[(long)i * a.ld + kc + e] : 0.0;
     }
     double dB[2][S], dS[2][S];
 #pragma unroll
     for (int e = 0; e < 2; e++) {
       split_value<S>(x[e] * sB / 128.0, 0, dB[e]);  // x * 2^(7-e) with e folded in sB
-      if (want_ds) {
-        if (kc + e < kx0_end) {
+      if (a.want_ds) {
+        if (kc + e < a.kx0_end) {
 #pragma unroll
           for (int p = 0; p < S; p++) dS[e][p] = 0.0;
         } else if (sS == sB) {
@@ -277,10 +308,10 @@ __global__ __launch_bounds__(256) void slice_psi_kernel(
 #pragma unroll
       for (int p = 0; p < S; p++) lt[p][(2 * cp + e) * PITCH + ii] = to_digit<DT>(dB[e][p]);
     }
-    if (want_ds) {
+    if (a.want_ds) {
 #pragma unroll
       for (int p = 0; p < S; p++) {
-        E *dst = DS + p * sds + kb_off<DT>(i, kc, np);
+        E *dst = DS + p * sds + kb_off<DT>(i, kc, a.np);
         if constexpr (ES == 1) {
           const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
                                                     (((unsigned)(int)dS[1][p] & 0xffu) << 8));
@@ -299,29 +330,46 @@ __global__ __launch_bounds__(256) void slice_psi_kernel(
   const int kk = t / 4, ib = (t % 4) * 16;
 #pragma unroll
   for (int p = 0; p < S; p++) {
-    E *dst = DB + p * sdb + kb_off<DT>(k0 + kk, i0 + ib, Kp);
+    E *dst = DB + p * sdb + kb_off<DT>(k0 + kk, i0 + ib, a.Kp);
 #pragma unroll
     for (int c = 0; c < 16; c += PER)
       *reinterpret_cast<i4v *>(dst + c) =
           *reinterpret_cast<const i4v *>(&lt[p][kk * PITCH + ib + c]);
   }
+  __syncthreads();
 }
 
-// One workgroup (256 threads) per row r of an n x n matrix M (row-major, or
-// its transpose when trans): the off-diagonal part X = M - diag(M) goes to
-// digits D[p][r][0..npk) (zero padded) with exponent f_r from the row's
-// off-diagonal maximum; the diagonal M[r][r] is kept in FP64 in diagv[r].
-// Rows r >= n are zero.
 template <int DT, int S>
-__global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restrict__ M, int n,
-                                                         int trans, typename DTraits<DT>::elem *D,
-                                                         int npk, int *fexp, double *diagv) {
-  const int r = blockIdx.x;
+__global__ __launch_bounds__(256) void slice_psi_kernel(oa_slice_args a, int total) {
+  const int item = single_item(total);
+  if (item < total) slice_psi_body<DT, S>(a, item);
+}
+
+template <int DT, int S>
+__global__ __launch_bounds__(256) void slice_psi_grouped(const oa_slice_args *__restrict__ t,
+                                                         const int *__restrict__ prefix,
+                                                         int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    slice_psi_body<DT, S>(t[i], item - prefix[i]);
+  });
+}
+
+// Item = row r of an n x n matrix M (row-major, or its transpose when trans):
+// the off-diagonal part X = M - diag(M) goes to digits D[p][r][0..npk) (zero
+// padded) with exponent f_r from the row's off-diagonal maximum; the diagonal
+// M[r][r] is kept in FP64 in diagv[r].  Rows r >= n are zero.
+template <int DT, int S>
+__device__ __forceinline__ void slice_rows_body(const oa_rows_args &a, int r) {
+  typedef typename DTraits<DT>::elem E;
+  E *D = reinterpret_cast<E *>(a.D);
+  const double *M = a.M;
+  const int n = a.n, npk = a.npk;
   __shared__ double red[256];
   double m = 0.0;
   if (r < n)
     for (int c = threadIdx.x; c < n; c += 256)
-      if (c != r) m = fmax(m, fabs(trans ? M[(long)c * n + r] : M[(long)r * n + c]));
+      if (c != r) m = fmax(m, fabs(a.trans ? M[(long)c * n + r] : M[(long)r * n + c]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -330,45 +378,38 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double *__restric
   }
   const int e = chunk_exp(red[0]);
   if (threadIdx.x == 0) {
-    fexp[r] = e;
-    diagv[r] = (r < n) ? M[(long)r * n + r] : 0.0;
+    a.fexp[r] = e;
+    a.diagv[r] = (r < n) ? M[(long)r * n + r] : 0.0;
   }
   for (int c = threadIdx.x; c < npk; c += 256) {
     double d[S];
     const bool in = (r < n && c < n && c != r);
-    const double x = in ? (trans ? M[(long)c * n + r] : M[(long)r * n + c]) : 0.0;
+    const double x = in ? (a.trans ? M[(long)c * n + r] : M[(long)r * n + c]) : 0.0;
     split_value<S>(x, e, d);
 #pragma unroll
     for (int p = 0; p < S; p++)
       D[(size_t)p * npk * npk + kb_off<DT>(r, c, npk)] = to_digit<DT>(d[p]);
   }
+  __syncthreads();
 }
 
-struct DigitArgs {
-  const void *X;  // k-blocked digits, rows m
-  long ldx, sx;   // rows of the layout and slice stride (elements)
-  const void *Y;  // k-blocked digits, rows n
-  long ldy, sy;
-  int M, N;       // output extent
-  int kc;         // contraction length per split (elements, multiple of DBK) or total
-  int kfull;      // contraction extent of the digit arrays (padded)
-  int ntiles, nz; // MODE 0: symmetric tiles, splits
-  double *part;   // MODE 0 output
-  const unsigned long long *chunk_max;  // MODE 0: per chunk scale source
-  // MODE 1
-  double *out;
-  long ldo;
-  double alpha, beta;
-  const int *fexp;                       // per output row exponent (T rows)
-  const double *diagv;                   // FP64 diagonal of T
-  const void *DB;                        // psi digits (rows Kp, contraction np), k-blocked
-  long dbs;                              // DB slice stride (elements)
-  int np;                                // DB contraction extent
-  int Kp;                                // DB rows
-  const unsigned long long *colmax;      // per chunk of output columns (psi chunks)
-  int kchunk;                            // psi chunk length for the column exponent
-};
+template <int DT, int S>
+__global__ __launch_bounds__(256) void slice_rows_kernel(oa_rows_args a, int total) {
+  const int item = single_item(total);
+  if (item < total) slice_rows_body<DT, S>(a, item);
+}
 
+template <int DT, int S>
+__global__ __launch_bounds__(256) void slice_rows_grouped(const oa_rows_args *__restrict__ t,
+                                                          const int *__restrict__ prefix,
+                                                          int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    slice_rows_body<DT, S>(t[i], item - prefix[i]);
+  });
+}
+
+// ------------------------------------------------------------ digit GEMM
 __device__ __forceinline__ void upper_tile_d(int tile, int T, int &bi, int &bj) {
   int b = 0, rem = tile;
   while (rem >= T - b) {
@@ -379,31 +420,31 @@ __device__ __forceinline__ void upper_tile_d(int tile, int T, int &bi, int &bj) 
   bj = b + rem;
 }
 
-// Level-grouped digit products of one 32-byte stage for a 32 x 64 wave tile:
-// acc[l] += sum over (p, q) with p < SA, q < SB, p + q = l < NL of
-// X_p Y_q^T.  Panels hold 128 rows x 32 bytes per slice with the two 16-byte
-// halves of row r swapped when bit 3 of r is set, which spreads any 16
-// consecutive lanes' 16-byte fragment reads over all 64 LDS banks.
-template <int DT, int SA, int SB, int NL>
+// Level-grouped digit products of one 32-byte stage for a 32 x (32*WJ) wave
+// tile: acc[l] += sum over (p, q) with p + q = l < NL of X_p Y_q^T.  Panels
+// hold TILE rows x 32 bytes per slice with the two 16-byte halves of row r
+// swapped when bit 3 of r is set, which spreads any 16 consecutive lanes'
+// 16-byte fragment reads over all 64 LDS banks.
+template <int DT, int NL, int WJ, int TILE>
 __device__ __forceinline__ void digit_stage(const char *Xs, const char *Ys, int xrow, int yrow,
                                             int lane,
-                                            typename DTraits<DT>::acc_t (&acc)[NL][2]) {
-  constexpr int PANEL = DBM * 32;
+                                            typename DTraits<DT>::acc_t (&acc)[NL][WJ]) {
+  constexpr int PANEL = TILE * 32;
   const int r = lane & 31;
   const int boff = r * 32 + 16 * ((lane >> 5) ^ ((r >> 3) & 1));
-  i4v a[SA];
+  i4v a[NL];
 #pragma unroll
-  for (int p = 0; p < SA; p++) a[p] = *reinterpret_cast<const i4v *>(Xs + p * PANEL + xrow * 32 + boff);
+  for (int p = 0; p < NL; p++) a[p] = *reinterpret_cast<const i4v *>(Xs + p * PANEL + xrow * 32 + boff);
 #pragma unroll
-  for (int q = 0; q < SB && q < NL; q++) {
-    i4v b[2];
+  for (int q = 0; q < NL; q++) {
+    i4v b[WJ];
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < WJ; j++)
       b[j] = *reinterpret_cast<const i4v *>(Ys + q * PANEL + (yrow + j * 32) * 32 + boff);
 #pragma unroll
-    for (int p = 0; p < SA && p + q < NL; p++)
+    for (int p = 0; p + q < NL; p++)
 #pragma unroll
-      for (int j = 0; j < 2; j++) acc[p + q][j] = mfma_digit<DT>(a[p], b[j], acc[p + q][j]);
+      for (int j = 0; j < WJ; j++) acc[p + q][j] = mfma_digit<DT>(a[p], b[j], acc[p + q][j]);
   }
 }
 
@@ -413,8 +454,8 @@ __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-// Digit of DB at element offset idx of a row of 32 (4 consecutive rows per
-// dword group for int8, per 8-byte group for 16-bit digits).
+// Digits of DB for 4 consecutive contraction rows (one dword group for int8,
+// one 8-byte group for 16-bit digits).
 template <int DT>
 __device__ __forceinline__ void load_digit_quad(const void *base, size_t off, double (&q)[4]) {
   if constexpr (DT == DT_I8) {
@@ -434,69 +475,77 @@ __device__ __forceinline__ void load_digit_quad(const void *base, size_t off, do
   }
 }
 
-// C[m][n] = sum over the contraction of level-grouped digit products.
+template <int DT, int S, int MODE, int NLB, int TILE>
+struct DigitCfg {
+  static constexpr int NL = (MODE == 0) ? S : NLB;  // slices in the product and levels kept
+  static constexpr int PANEL = TILE * 32;           // bytes of one slice panel per stage
+  static constexpr int STAGE = 2 * NL * PANEL;      // X and Y slices of one stage
+  static constexpr int NBUF = 3;                    // LDS ring depth
+  static constexpr int SMEM = NBUF * STAGE;
+  static constexpr int NW = TileCfg<TILE>::NW;
+  static constexpr int NTHR = 64 * NW;
+};
+
+// C[m][n] = sum over the contraction of level-grouped digit products, for one
+// output tile (item `logical`).
 //   MODE 0: symmetric split-K partial tiles of S; S slices, levels < S.
 //   MODE 1: back-multiply with T = D + X: out[m][n] = alpha (D_m psi~[m][n]
-//           + 2^(f_m + e(n) - 14) C) + beta out, where C uses S-1 slices of
-//           X and of psi (levels < S-1) and psi~ is psi rebuilt from all S
+//           + 2^(f_m + e(n) - 14) C) + beta out, where C uses NLB slices of
+//           X and of psi (levels < NLB) and psi~ is psi rebuilt from all S
 //           digits of DB.
-// 8 waves as 4 (rows) x 2 (cols); wave tile 32 x 64; double-buffered LDS.
-template <int DT, int S, int MODE, int NLB = S - 1>
-__global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
+// The X and Y panels of three stages rotate through LDS, filled by
+// direct-to-LDS loads (global_load_lds_dwordx4) two stages ahead.
+template <int DT, int S, int MODE, int NLB, int TILE>
+__device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, char *smem) {
   typedef typename DTraits<DT>::acc_t ACC;
-  constexpr int NL = (MODE == 0) ? S : NLB;  // slices in the product and levels kept
+  typedef DigitCfg<DT, S, MODE, NLB, TILE> C;
+  constexpr int NL = C::NL;
   constexpr int ES = DTraits<DT>::ES;
-  constexpr int SK = 32 / ES;                 // contraction elements per stage
-  constexpr int PANEL = DBM * 32;             // bytes of one slice panel per stage
-  constexpr int STAGE = 2 * NL * PANEL;       // X and Y slices of one stage
-  constexpr int NBUF = 3;                     // LDS ring depth
-  constexpr int NI = 2 * NL * 4;              // 1 KB wave-loads per stage
-  constexpr int PERW = NI / 8;                // per wave (8 waves)
+  constexpr int SK = 32 / ES;             // contraction elements per stage
+  constexpr int PANEL = C::PANEL;
+  constexpr int STAGE = C::STAGE;
+  constexpr int NBUF = C::NBUF;
+  constexpr int NW = C::NW;
+  constexpr int WJ = TileCfg<TILE>::WJ;
+  constexpr int NQ = TILE / 32;           // 1 KB wave-loads per slice panel
+  constexpr int NI = 2 * NL * NQ;         // wave-loads per stage
+  constexpr int PERW = NI / NW;           // per wave
+  static_assert(NI % NW == 0, "stage loads must split evenly over the waves");
   // fp32 accumulation is exact only below 2^24 (level sums < 6 * 127^2 =
   // 96774 per k for S <= 6): promote every PROMOTE stages (128 k) into one
   // FP64 sum per output
   constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 128 / SK;
-  __shared__ __attribute__((aligned(1024))) char smem[NBUF * STAGE];
 
-  int bi, bj, z = 0, tile;
+  int bi, bj, z = 0, tile = logical;
   if (MODE == 0) {
-    const int T = cdv(a.M, DBM);
-    const int b = blockIdx.x, nb = gridDim.x, per = nb >> 3;
-    const int logical = (b & 7) * per + (b >> 3);
-    if (logical >= a.ntiles * a.nz) return;
     tile = logical % a.ntiles;
     z = logical / a.ntiles;
-    upper_tile_d(tile, T, bi, bj);
+    upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
   } else {
-    const int tm = cdv(a.M, DBM);
-    const int b = blockIdx.x;
-    const int nb = gridDim.x;
-    const int per = nb >> 3;
-    const int logical = (b & 7) * per + (b >> 3);
-    if (logical >= tm * cdv(a.N, DBM)) return;
+    const int tm = cdv(a.M, TILE);
     bi = logical % tm;
     bj = logical / tm;
-    tile = logical;
   }
-  const int m0 = bi * DBM, n0 = bj * DBM;
+  const int m0 = bi * TILE, n0 = bj * TILE;
   const int ka = (MODE == 0) ? z * a.kc : 0;
   const int kb = (MODE == 0) ? min(a.kfull, ka + a.kc) : a.kfull;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int xrow = (w >> 1) * 32, yrow = (w & 1) * 64;
+  const int xrow = (w >> 1) * 32, yrow = (w & 1) * 32 * WJ;
 
-  ACC acc[NL][2];
-  double facc[2][16];
+  ACC acc[NL][WJ];
+  double facc[WJ][16];
 #pragma unroll
   for (int l = 0; l < NL; l++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[l][j] = ACC{};
+    for (int j = 0; j < WJ; j++) acc[l][j] = ACC{};
 #pragma unroll
-  for (int j = 0; j < 2; j++)
+  for (int j = 0; j < WJ; j++)
 #pragma unroll
     for (int v = 0; v < 16; v++) facc[j][v] = 0.0;
 
   const char *Xg = reinterpret_cast<const char *>(a.X);
   const char *Yg = reinterpret_cast<const char *>(a.Y);
+  const size_t sx = a.sx, sy = a.sy, ldx = a.ldx, ldy = a.ldy;
   // Wave-load i of a stage: operand o, slice p, quarter q (rows 32q..32q+31).
   // Lane L fills LDS slot q*64 + L of the panel, i.e. row r = (q*64+L)/2,
   // stored half hs = L%2, which holds global half hs ^ ((r >> 3) & 1).
@@ -505,12 +554,12 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
     char *buf = smem + (st % NBUF) * STAGE;
 #pragma unroll
     for (int j = 0; j < PERW; j++) {
-      const int i = w + 8 * j;
-      const int o = i / (NL * 4), p = (i / 4) % NL, q = i % 4;
+      const int i = w + NW * j;
+      const int o = i / (NL * NQ), p = (i / NQ) % NL, q = i % NQ;
       const int pos = q * 64 + lane, r = pos >> 1, h = (pos & 1) ^ ((r >> 3) & 1);
       const char *g = (o == 0)
-          ? Xg + ((size_t)p * a.sx + ((size_t)kblk * a.ldx + m0 + r) * SK) * ES + h * 16
-          : Yg + ((size_t)p * a.sy + ((size_t)kblk * a.ldy + n0 + r) * SK) * ES + h * 16;
+          ? Xg + ((size_t)p * sx + ((size_t)kblk * ldx + m0 + r) * SK) * ES + h * 16
+          : Yg + ((size_t)p * sy + ((size_t)kblk * ldy + n0 + r) * SK) * ES + h * 16;
       char *l = buf + o * NL * PANEL + p * PANEL + q * 1024;
       __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
     }
@@ -524,10 +573,10 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
     __syncthreads();
     if (s + 2 < nst) issue(s + 2);
     const char *cur = smem + (s % NBUF) * STAGE;
-    digit_stage<DT, NL, NL, NL>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
+    digit_stage<DT, NL, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
     if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
 #pragma unroll
-      for (int j = 0; j < 2; j++)
+      for (int j = 0; j < WJ; j++)
 #pragma unroll
         for (int v = 0; v < 16; v++) {
           double t = 0.0;
@@ -538,13 +587,13 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
 #pragma unroll
       for (int l = 0; l < NL; l++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[l][j] = ACC{};
+        for (int j = 0; j < WJ; j++) acc[l][j] = ACC{};
     }
   }
 
   // epilogue: combine levels in FP64
 #pragma unroll
-  for (int j = 0; j < 2; j++) {
+  for (int j = 0; j < WJ; j++) {
     const int ln = yrow + j * 32 + (lane & 31);
     const int nn = n0 + ln;
     // MODE 1: psi~[m][nn] for this lane's 16 rows, rebuilt from all S digits
@@ -580,7 +629,7 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
       const int lm = xrow + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
       if (MODE == 0) {
         const int e = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
-        a.part[((size_t)z * a.ntiles + tile) * (DBM * DBM) + lm * DBM + ln] =
+        a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
             ldexp(val, 2 * (e - 7));
       } else {
         const int m = m0 + lm;
@@ -593,47 +642,77 @@ __global__ __launch_bounds__(DNT) void digit_nt_kernel(DigitArgs a) {
       }
     }
   }
+  // the next item's first loads overwrite the ring
+  __syncthreads();
 }
 
-// Fixed-order sum of the split-K partial tiles + FP64 kx0 term.
-__global__ __launch_bounds__(256) void split_reduce_kernel(
-    const double *__restrict__ part, int n, int nz, double alpha, const double *__restrict__ psi,
-    long ld, int kx0_end, double *S, double diag, unsigned long long *maxdev) {
-  const int T = cdv(n, DBM);
+template <int DT, int S, int MODE, int NLB, int TILE>
+__global__ __launch_bounds__(64 * TileCfg<TILE>::NW) void digit_nt_kernel(
+    oa_digit_args a, int total) {
+  __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
+  const int item = single_item(total);
+  if (item < total) digit_body<DT, S, MODE, NLB, TILE>(a, item, smem);
+}
+
+template <int DT, int S, int MODE, int NLB, int TILE>
+__global__ __launch_bounds__(64 * TileCfg<TILE>::NW) void digit_nt_grouped(
+    const oa_digit_args *__restrict__ t, const int *__restrict__ prefix, int count) {
+  __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    digit_body<DT, S, MODE, NLB, TILE>(t[i], item - prefix[i], smem);
+  });
+}
+
+// ------------------------------------------------------------ reduce
+// Item = 256 consecutive elements of the upper-triangle partial tiles: fixed
+// order sum over the splits, the FP64 kx0 term (weight 1/2), mirror,
+// max|S - d I|.
+template <int TILE>
+__device__ __forceinline__ void split_reduce_body(const oa_sreduce_args &a, int item) {
+  const int n = a.n;
+  const int T = cdv(n, TILE);
   const int ntiles = T * (T + 1) / 2;
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  const int tile = (int)(idx / (DBM * DBM));
+  const long idx = (long)item * 256 + threadIdx.x;
+  const int tile = (int)(idx / (TILE * TILE));
   if (tile >= ntiles) return;
-  const int e = (int)(idx % (DBM * DBM));
-  const int lm = e / DBM, ln = e % DBM;
+  const int e = (int)(idx % (TILE * TILE));
+  const int lm = e / TILE, ln = e % TILE;
   int bi, bj;
   upper_tile_d(tile, T, bi, bj);
   if (bi == bj && ln < lm) return;
-  const int i = bi * DBM + lm, j = bj * DBM + ln;
+  const int i = bi * TILE + lm, j = bj * TILE + ln;
   if (i >= n || j >= n) return;
   double s = 0.0;
-  for (int z = 0; z < nz; z++) s += part[((long)z * ntiles + tile) * (DBM * DBM) + e];
+  for (int z = 0; z < a.nz; z++) s += a.part[((long)z * ntiles + tile) * (TILE * TILE) + e];
   double g = 0.0;
-  for (int k = 0; k < kx0_end; k++) g += psi[(long)i * ld + k] * psi[(long)j * ld + k];
-  s = alpha * s + 0.5 * alpha * g;
-  S[(long)i * n + j] = s;
-  S[(long)j * n + i] = s;
-  if (maxdev) {
-    const double dv = fabs(s - (i == j ? diag : 0.0));
-    atomicMax(maxdev, (unsigned long long)__double_as_longlong(dv));
+  for (int k = 0; k < a.kx0_end; k++) g += a.psi[(long)i * a.ld + k] * a.psi[(long)j * a.ld + k];
+  s = a.alpha * s + 0.5 * a.alpha * g;
+  a.S[(long)i * n + j] = s;
+  a.S[(long)j * n + i] = s;
+  if (a.maxdev) {
+    const double dv = fabs(s - (i == j ? a.diag : 0.0));
+    atomicMax(a.maxdev, (unsigned long long)__double_as_longlong(dv));
   }
 }
 
-// ---------------------------------------------------------------- host plan
-struct SplitPlan {
-  int dt, S, es;
-  int np, Kp;      // padded rows (multiple of DBM) and columns (multiple of 64 and DBK)
-  int kchunk, nz;  // psi chunks (exponents and split-K)
-  int ntiles;
-  size_t off_ds, off_db, off_dt, off_maxS, off_maxB, off_scS, off_scB, off_fexp, off_diag,
-      off_part, off_md, total;
-};
+template <int TILE>
+__global__ __launch_bounds__(256) void split_reduce_kernel(oa_sreduce_args a, int total) {
+  const int item = single_item(total);
+  if (item < total) split_reduce_body<TILE>(a, item);
+}
 
+template <int TILE>
+__global__ __launch_bounds__(256) void split_reduce_grouped(const oa_sreduce_args *__restrict__ t,
+                                                            const int *__restrict__ prefix,
+                                                            int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    split_reduce_body<TILE>(t[i], item - prefix[i]);
+  });
+}
+
+// ---------------------------------------------------------------- host plan
 bool decode_prec(int prec, int &dt, int &S) {
   const int base = prec & 0xff;
   S = (prec >> 8) & 0xf;
@@ -662,26 +741,36 @@ long level_bound(int S) {
   return best;
 }
 
-SplitPlan make_plan(int prec, int n, int K) {
-  SplitPlan pl{};
-  decode_prec(prec, pl.dt, pl.S);
-  pl.es = (pl.dt == DT_I8) ? 1 : 2;
-  pl.np = cdv(n, DBM) * DBM;
-  pl.Kp = cdv(K, DBM) * DBM;
-  const int T = cdv(n, DBM);
-  pl.ntiles = T * (T + 1) / 2;
+// Chunk length from (n, K) alone, so that every tile shape and launch form
+// sees the same digits.
+int chunk_len(int dt, int S, int n, int K) {
+  const int T128 = cdv(n, 128);
+  const long t128 = (long)T128 * (T128 + 1) / 2;
+  const int K128 = cdv(K, 128) * 128;
   // The chunk is both the split-K unit and the scale block: short chunks
   // keep each chunk's exponent close to its own coefficients (the envelope
   // falls steeply with |G|), and the int32 sums stay exact below the cap.
-  long cap = (pl.dt == DT_I8) ? (2147483647L / level_bound(pl.S)) : (1L << 30);
+  long cap = (dt == DT_I8) ? (2147483647L / level_bound(S)) : (1L << 30);
   cap = cap / DBK * DBK;
   long target = KCHUNK_TARGET;
-  // enough workgroups for small problems: at least ~256 of them
-  while (target > 512 && (long)pl.ntiles * cdv(pl.Kp, (int)target) < 256) target /= 2;
+  // enough workgroups for one instance alone: at least ~256 of them
+  while (target > 512 && t128 * cdv(K128, (int)target) < 256) target /= 2;
   int kchunk = (int)(target < cap ? target : cap);
-  kchunk = max(DBK, kchunk / DBK * DBK);
-  pl.kchunk = kchunk;
-  pl.nz = cdv(pl.Kp, kchunk);
+  return max(DBK, kchunk / DBK * DBK);
+}
+
+oa_split_plan make_plan(int prec, int n, int K, int tile) {
+  oa_split_plan pl{};
+  decode_prec(prec, pl.dt, pl.S);
+  pl.tile = tile;
+  pl.es = (pl.dt == DT_I8) ? 1 : 2;
+  pl.np = cdv(n, tile) * tile;
+  const int kq = tile > SLT ? tile : SLT;
+  pl.Kp = cdv(K, kq) * kq;
+  const int T = pl.np / tile;
+  pl.ntiles = T * (T + 1) / 2;
+  pl.kchunk = chunk_len(pl.dt, pl.S, n, K);
+  pl.nz = cdv(pl.Kp, pl.kchunk);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t o = off;
@@ -693,43 +782,77 @@ SplitPlan make_plan(int prec, int n, int K) {
   pl.off_dt = take((size_t)pl.S * pl.np * pl.np * pl.es);
   pl.off_maxS = take((size_t)pl.nz * 8);
   pl.off_maxB = take((size_t)pl.nz * 8);
-  pl.off_scS = take((size_t)pl.nz * 8);
-  pl.off_scB = take((size_t)pl.nz * 8);
   pl.off_fexp = take((size_t)pl.np * 4);
   pl.off_diag = take((size_t)pl.np * 8);
   pl.off_md = take(8);
-  pl.off_part = take((size_t)pl.nz * pl.ntiles * DBM * DBM * 8);
+  pl.off_part = take((size_t)pl.nz * pl.ntiles * tile * tile * 8);
   pl.total = off;
   return pl;
 }
 
-template <int DT, int S>
-void launch_slice_psi(hipStream_t s, const SplitPlan &pl, const double *psi, long ld, int n, int K,
-                      int kx0_end, bool want_ds, char *ws) {
-  typedef typename DTraits<DT>::elem E;
-  auto *maxS = reinterpret_cast<unsigned long long *>(ws + pl.off_maxS);
-  auto *maxB = reinterpret_cast<unsigned long long *>(ws + pl.off_maxB);
-  double *scS = reinterpret_cast<double *>(ws + pl.off_scS);
-  double *scB = reinterpret_cast<double *>(ws + pl.off_scB);
-  hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, maxS, pl.nz);
-  hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, maxB, pl.nz);
-  hipLaunchKernelGGL(chunk_max_kernel, dim3(pl.nz, cdv(n, 8)), dim3(256), 0, s, psi, ld, n, K,
-                     kx0_end, pl.kchunk, maxS, maxB);
-  hipLaunchKernelGGL(chunk_scale_kernel, dim3(cdv(pl.nz, 256)), dim3(256), 0, s,
-                     (const unsigned long long *)maxS, (const unsigned long long *)maxB, pl.nz,
-                     scS, scB);
-  const int vec = (((uintptr_t)psi & 15) == 0 && (ld % 2) == 0) ? 1 : 0;
-  hipLaunchKernelGGL((slice_psi_kernel<DT, S>), dim3(pl.Kp / 64, pl.np / 64), dim3(256), 0, s,
-                     psi, ld, n, K, kx0_end, pl.kchunk, (const double *)scS, (const double *)scB,
-                     reinterpret_cast<E *>(ws + pl.off_ds), reinterpret_cast<E *>(ws + pl.off_db),
-                     pl.np, pl.Kp, want_ds ? 1 : 0, vec);
+// ---------------------------------------------------------------- launches
+int device_cus() {
+  static thread_local int cached_dev = -1, cached_cus = 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev != cached_dev) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cached_dev = dev;
+    cached_cus = cus;
+  }
+  return cached_cus;
 }
 
-template <int DT, int S>
-int run_overlap(hipStream_t s, const SplitPlan &pl, int n, int K, const double *psi, long ld,
-                int kx0_end, double alpha, double *Sout, double diag, double *maxdev, char *ws) {
-  launch_slice_psi<DT, S>(s, pl, psi, ld, n, K, kx0_end, true, ws);
-  DigitArgs a{};
+// Grid of a launch over `total` items: one workgroup per item (a multiple of
+// 8) for single-instance launches; for grouped launches at most the number
+// of co-resident workgroups, so the grid is persistent.
+template <class K>
+int grid_for(K kernel, int threads, int total, bool persistent) {
+  int g = cdv(max(total, 1), 8) * 8;
+  if (persistent) {
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    const int cap = cdv(device_cus() * per_cu, 8) * 8;
+    g = min(g, cap);
+  }
+  return g;
+}
+
+#define OA_LAUNCH_ITEMS(KERN, THREADS, TOTAL, STREAM, ARGS)                                    \
+  do {                                                                                       \
+    const int _t = (TOTAL);                                                                  \
+    const int _g = grid_for(KERN, THREADS, _t, false);                                       \
+    hipLaunchKernelGGL(KERN, dim3(_g), dim3(THREADS), 0, STREAM, ARGS, _t);                  \
+  } while (0)
+
+#define OA_LAUNCH_GROUPED(KERN, THREADS, STREAM, G, T)                                       \
+  do {                                                                                       \
+    const int _g = grid_for(KERN, THREADS, (G).total, true);                                 \
+    hipLaunchKernelGGL(KERN, dim3(_g), dim3(THREADS), 0, STREAM,                             \
+                       reinterpret_cast<const T *>((G).args), (G).prefix, (G).count);        \
+  } while (0)
+
+// Argument structs of one instance for every phase.
+void fill_inst(const oa_split_plan &pl, int n, int K, const double *psi, long ld, int kx0_end,
+               double alpha, double *Sout, const double *T, int transT, double beta,
+               double *out, long ldout, char *ws, unsigned long long *maxS,
+               unsigned long long *maxB, oa_split_inst_args *r) {
+  const int tile = pl.tile;
+  if (!maxS) maxS = reinterpret_cast<unsigned long long *>(ws + pl.off_maxS);
+  if (!maxB) maxB = reinterpret_cast<unsigned long long *>(ws + pl.off_maxB);
+  const int vec = (((uintptr_t)psi & 15) == 0 && (ld % 2) == 0) ? 1 : 0;
+  r->cm = oa_chunkmax_args{psi, ld, n, K, kx0_end, pl.kchunk, pl.nz, 0, maxS, maxB};
+  r->items_cm = pl.nz * cdv(n, 8);
+  r->sl = oa_slice_args{psi, ld, n, K, kx0_end, pl.kchunk, maxS, maxB, ws + pl.off_ds,
+                        ws + pl.off_db, pl.np, pl.Kp, 1, vec};
+  r->items_sl = (pl.Kp / SLT) * (pl.np / SLT);
+  oa_digit_args &a = r->d0;
+  a = oa_digit_args{};
   a.X = ws + pl.off_ds;
   a.ldx = pl.np;
   a.sx = (long)pl.np * pl.Kp;
@@ -743,58 +866,87 @@ int run_overlap(hipStream_t s, const SplitPlan &pl, int n, int K, const double *
   a.ntiles = pl.ntiles;
   a.nz = pl.nz;
   a.part = reinterpret_cast<double *>(ws + pl.off_part);
-  a.chunk_max = reinterpret_cast<const unsigned long long *>(ws + pl.off_maxS);
-  hipLaunchKernelGGL((digit_nt_kernel<DT, S, 0>), dim3(cdv(pl.ntiles * pl.nz, 8) * 8), dim3(DNT),
-                     0, s, a);
+  a.chunk_max = maxS;
+  r->items_d0 = pl.ntiles * pl.nz;
+  r->rd = oa_sreduce_args{a.part, n, pl.nz, alpha, psi, ld, kx0_end, 0, Sout, 0.0, nullptr};
+  r->items_rd = cdv(pl.ntiles * tile * tile, 256);
+  r->rows = oa_rows_args{T, n, transT, ws + pl.off_dt, pl.np, 0,
+                         reinterpret_cast<int *>(ws + pl.off_fexp),
+                         reinterpret_cast<double *>(ws + pl.off_diag)};
+  r->items_rows = pl.np;
+  oa_digit_args &b = r->d1;
+  b = oa_digit_args{};
+  b.X = ws + pl.off_dt;
+  b.ldx = pl.np;
+  b.sx = (long)pl.np * pl.np;
+  b.Y = ws + pl.off_db;
+  b.ldy = pl.Kp;
+  b.sy = (long)pl.Kp * pl.np;
+  b.M = n;
+  b.N = K;
+  b.kc = pl.np;
+  b.kfull = pl.np;
+  b.out = out;
+  b.ldo = ldout;
+  b.alpha = 1.0;
+  b.beta = beta;
+  b.fexp = reinterpret_cast<const int *>(ws + pl.off_fexp);
+  b.diagv = reinterpret_cast<const double *>(ws + pl.off_diag);
+  b.DB = ws + pl.off_db;
+  b.dbs = (long)pl.Kp * pl.np;
+  b.np = pl.np;
+  b.Kp = pl.Kp;
+  b.colmax = maxB;
+  b.kchunk = pl.kchunk;
+  r->items_d1 = cdv(n, tile) * cdv(K, tile);
+}
+
+template <int DT, int S>
+void launch_slice_psi(hipStream_t s, const oa_split_plan &pl, const oa_split_inst_args &r,
+                      bool want_ds) {
+  hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, r.cm.maxS, (long)pl.nz);
+  hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, r.cm.maxB, (long)pl.nz);
+  OA_LAUNCH_ITEMS(chunk_max_kernel, 256, r.items_cm, s, r.cm);
+  oa_slice_args sl = r.sl;
+  sl.want_ds = want_ds ? 1 : 0;
+  OA_LAUNCH_ITEMS((slice_psi_kernel<DT, S>), 256, r.items_sl, s, sl);
+}
+
+template <int DT, int S>
+int run_overlap(hipStream_t s, const oa_split_plan &pl, int n, int K, const double *psi,
+                long ld, int kx0_end, double alpha, double *Sout, double diag, double *maxdev,
+                char *ws) {
+  oa_split_inst_args r;
+  fill_inst(pl, n, K, psi, ld, kx0_end, alpha, Sout, nullptr, 0, 0.0, nullptr, 0, ws, nullptr,
+            nullptr, &r);
+  launch_slice_psi<DT, S>(s, pl, r, true);
+  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 0, S - 1, 128>),
+                  (DigitCfg<DT, S, 0, S - 1, 128>::NTHR), r.items_d0, s, r.d0);
   unsigned long long *md = reinterpret_cast<unsigned long long *>(maxdev);
-  if (md) hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, md, 1);
-  const long total = (long)pl.ntiles * DBM * DBM;
-  hipLaunchKernelGGL(split_reduce_kernel, dim3((unsigned)cdv((int)total, 256)), dim3(256), 0, s,
-                     (const double *)a.part, n, pl.nz, alpha, psi, ld, kx0_end, Sout, diag, md);
+  if (md) hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, md, 1L);
+  r.rd.diag = diag;
+  r.rd.maxdev = md;
+  OA_LAUNCH_ITEMS(split_reduce_kernel<128>, 256, r.items_rd, s, r.rd);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
 template <int DT, int S, int NLB>
-int run_backmul(hipStream_t s, const SplitPlan &pl, int n, int K, const double *M, int transM,
+int run_backmul(hipStream_t s, const oa_split_plan &pl, int n, int K, const double *M, int transM,
                 const double *psi, long ldpsi, int kx0_end, bool reuse, double alpha,
                 double beta, double *out, long ldout, char *ws) {
-  typedef typename DTraits<DT>::elem E;
-  if (!reuse) launch_slice_psi<DT, S>(s, pl, psi, ldpsi, n, K, kx0_end, false, ws);
-  hipLaunchKernelGGL((slice_rows_kernel<DT, NLB>), dim3(pl.np), dim3(256), 0, s,
-                     M, n, transM, reinterpret_cast<E *>(ws + pl.off_dt), pl.np,
-                     reinterpret_cast<int *>(ws + pl.off_fexp),
-                     reinterpret_cast<double *>(ws + pl.off_diag));
-  DigitArgs a{};
-  a.X = ws + pl.off_dt;
-  a.ldx = pl.np;
-  a.sx = (long)pl.np * pl.np;
-  a.Y = ws + pl.off_db;
-  a.ldy = pl.Kp;
-  a.sy = (long)pl.Kp * pl.np;
-  a.M = n;
-  a.N = K;
-  a.kc = pl.np;
-  a.kfull = pl.np;
-  a.out = out;
-  a.ldo = ldout;
-  a.alpha = alpha;
-  a.beta = beta;
-  a.fexp = reinterpret_cast<const int *>(ws + pl.off_fexp);
-  a.diagv = reinterpret_cast<const double *>(ws + pl.off_diag);
-  a.DB = ws + pl.off_db;
-  a.dbs = (long)pl.Kp * pl.np;
-  a.np = pl.np;
-  a.Kp = pl.Kp;
-  a.colmax = reinterpret_cast<const unsigned long long *>(ws + pl.off_maxB);
-  a.kchunk = pl.kchunk;
-  const int tiles = cdv(n, DBM) * cdv(K, DBM);
-  hipLaunchKernelGGL((digit_nt_kernel<DT, S, 1, NLB>), dim3(cdv(tiles, 8) * 8), dim3(DNT), 0, s,
-                     a);
+  oa_split_inst_args r;
+  fill_inst(pl, n, K, psi, ldpsi, kx0_end, 1.0, nullptr, M, transM, beta, out, ldout, ws,
+            nullptr, nullptr, &r);
+  r.d1.alpha = alpha;
+  if (!reuse) launch_slice_psi<DT, S>(s, pl, r, false);
+  OA_LAUNCH_ITEMS((slice_rows_kernel<DT, NLB>), 256, r.items_rows, s, r.rows);
+  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 1, NLB, 128>), (DigitCfg<DT, S, 1, NLB, 128>::NTHR),
+                  r.items_d1, s, r.d1);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
 template <int DT>
-int dispatch_overlap(hipStream_t st, const SplitPlan &pl, int n, int K, const double *psi,
+int dispatch_overlap(hipStream_t st, const oa_split_plan &pl, int n, int K, const double *psi,
                      long ld, int kx0_end, double alpha, double *S, double diag, double *md,
                      char *ws) {
   switch (pl.S) {
@@ -808,7 +960,7 @@ int dispatch_overlap(hipStream_t st, const SplitPlan &pl, int n, int K, const do
 }
 
 template <int DT>
-int dispatch_backmul(hipStream_t st, const SplitPlan &pl, bool lean, int n, int K,
+int dispatch_backmul(hipStream_t st, const oa_split_plan &pl, bool lean, int n, int K,
                      const double *M, int transM, const double *psi, long ldpsi, int kx0_end,
                      bool reuse, double alpha, double beta, double *out, long ldout, char *ws) {
 #define OA_BM(SS, NL)                                                                        \
@@ -825,12 +977,94 @@ int dispatch_backmul(hipStream_t st, const SplitPlan &pl, bool lean, int n, int 
   return OA_ERR_UNSUPPORTED;
 }
 
+// ------------------------------------------------------------ grouped launches
+template <int DT, int S, int NLB, int TILE>
+int grouped_phase(hipStream_t s, int phase, const oa_group &g) {
+  switch (phase) {
+    case OA_SP_CHUNKMAX:
+      OA_LAUNCH_GROUPED(chunk_max_grouped, 256, s, g, oa_chunkmax_args);
+      break;
+    case OA_SP_SLICE:
+      OA_LAUNCH_GROUPED((slice_psi_grouped<DT, S>), 256, s, g, oa_slice_args);
+      break;
+    case OA_SP_DIGIT0:
+      OA_LAUNCH_GROUPED((digit_nt_grouped<DT, S, 0, S - 1, TILE>),
+                        (DigitCfg<DT, S, 0, S - 1, TILE>::NTHR), s, g, oa_digit_args);
+      break;
+    case OA_SP_REDUCE:
+      OA_LAUNCH_GROUPED(split_reduce_grouped<TILE>, 256, s, g, oa_sreduce_args);
+      break;
+    case OA_SP_ROWS:
+      OA_LAUNCH_GROUPED((slice_rows_grouped<DT, NLB>), 256, s, g, oa_rows_args);
+      break;
+    case OA_SP_DIGIT1:
+      OA_LAUNCH_GROUPED((digit_nt_grouped<DT, S, 1, NLB, TILE>),
+                        (DigitCfg<DT, S, 1, NLB, TILE>::NTHR), s, g, oa_digit_args);
+      break;
+    default:
+      return OA_ERR_UNSUPPORTED;
+  }
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+template <int DT, int TILE>
+int grouped_s(hipStream_t s, int S, bool lean, int phase, const oa_group &g) {
+#define OA_GP(SS, NL) return grouped_phase<DT, SS, NL, TILE>(s, phase, g);
+  switch (S) {
+    case 2: OA_GP(2, 1)
+    case 3: if (lean) { OA_GP(3, 1) } else { OA_GP(3, 2) }
+    case 4: if (lean) { OA_GP(4, 2) } else { OA_GP(4, 3) }
+    case 5: if (lean) { OA_GP(5, 3) } else { OA_GP(5, 4) }
+    case 6: if (lean) { OA_GP(6, 4) } else { OA_GP(6, 5) }
+  }
+#undef OA_GP
+  return OA_ERR_UNSUPPORTED;
+}
+
+template <int DT>
+int grouped_dt(hipStream_t s, int S, bool lean, int tile, int phase, const oa_group &g) {
+  if (tile == 128) return grouped_s<DT, 128>(s, S, lean, phase, g);
+  if (tile == 64) return grouped_s<DT, 64>(s, S, lean, phase, g);
+  return OA_ERR_UNSUPPORTED;
+}
+
 }  // namespace
+
+oa_split_plan oa_split_plan_for(int prec, int n, int K, int tile) {
+  return make_plan(prec & 0xffff, n, K, tile);
+}
+
+void oa_split_inst_fill(const oa_split_plan &pl, int prec, int n, int K, const double *psi,
+                        long ld, int kx0_end, double alpha, double *S, const double *T,
+                        int transT, double beta, double *out, long ldout, char *ws,
+                        unsigned long long *maxS, unsigned long long *maxB,
+                        oa_split_inst_args *a) {
+  (void)prec;
+  fill_inst(pl, n, K, psi, ld, kx0_end, alpha, S, T, transT, beta, out, ldout, ws, maxS, maxB,
+            a);
+}
+
+int oa_split_grouped(hipStream_t s, int prec, int tile, int phase, const oa_group &g) {
+  int dt, S;
+  if (!decode_prec(prec & 0xffff, dt, S)) return OA_ERR_UNSUPPORTED;
+  if (g.count <= 0 || g.total <= 0) return OA_SUCCESS;
+  const bool lean = (prec & OA_PREC_LEAN_BACKMUL) != 0;
+  if (dt == DT_I8) return grouped_dt<DT_I8>(s, S, lean, tile, phase, g);
+  if (dt == DT_F16) return grouped_dt<DT_F16>(s, S, lean, tile, phase, g);
+  return grouped_dt<DT_BF16>(s, S, lean, tile, phase, g);
+}
+
+int oa_fill_u64(hipStream_t s, unsigned long long *p, long n) {
+  if (n <= 0) return OA_SUCCESS;
+  const long nb = (n + 255) / 256;
+  hipLaunchKernelGGL(fill_u64, dim3((unsigned)(nb > 1024 ? 1024 : nb)), dim3(256), 0, s, p, n);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
 
 size_t oa_split_ws_bytes(int prec, int n, int K) {
   int dt, S;
   if ((prec & 0xff) == OA_PREC_FP64 || !decode_prec(prec & 0xffff, dt, S)) return 0;
-  return make_plan(prec & 0xffff, n, K).total;
+  return make_plan(prec & 0xffff, n, K, 128).total;
 }
 
 int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K, const double *psi, long ld,
@@ -838,7 +1072,7 @@ int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K, const double 
                             void *ws, size_t ws_bytes) {
   int dt, Sl;
   if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
-  const SplitPlan pl = make_plan(prec & 0xffff, n, K);
+  const oa_split_plan pl = make_plan(prec & 0xffff, n, K, 128);
   if (pl.total > ws_bytes) return OA_ERR_CAPACITY;
   char *w = (char *)ws;
   if (dt == DT_I8) return dispatch_overlap<DT_I8>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
@@ -851,7 +1085,7 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double 
                             double *out, long ldout, void *ws, size_t ws_bytes) {
   int dt, Sl;
   if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
-  const SplitPlan pl = make_plan(prec & 0xffff, n, K);
+  const oa_split_plan pl = make_plan(prec & 0xffff, n, K, 128);
   if (pl.total > ws_bytes) return OA_ERR_CAPACITY;
   const bool reuse = (prec & OA_PREC_REUSE_DIGITS) != 0;
   const bool lean = (prec & OA_PREC_LEAN_BACKMUL) != 0;

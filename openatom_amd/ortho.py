@@ -180,6 +180,100 @@ class Handle:
         return S, T, out, None, None
 
 
+class Batch:
+    """Many independent instances (beads, k-points, tempering replicas) of
+    uneven size advanced together: each phase of the step is one persistent
+    grouped launch over all of them (oa_batch_*).  Each instance's results
+    equal Handle.ortho_step on that instance alone, bit for bit."""
+
+    def __init__(self, handle: Handle | None = None):
+        self.handle = handle or default_handle()
+        b = ctypes.c_void_p()
+        check(_lib.lib().oa_batch_create(self.handle.h, ctypes.byref(b)), "oa_batch_create")
+        self.b = b
+        self._keep = []
+        self.count = 0
+
+    def plan(self, psis, outs=None, nkx0=1, nspin=1, tol: float = 1e-15, maxit: int = 1000,
+             prec="fp64", S=None, T=None):
+        """psis: list of (n_i, 2*npw_i) float64 device tensors; outs / S / T:
+        lists of outputs (allocated when None).  nkx0 / nspin / tol / maxit:
+        scalars or per-instance lists."""
+        count = len(psis)
+        per = lambda v, i: v[i] if isinstance(v, (list, tuple)) else v  # noqa: E731
+        outs = outs or [torch.empty_like(p) for p in psis]
+        S = S or [torch.empty((p.shape[0], p.shape[0]), dtype=torch.float64, device=p.device)
+                  for p in psis]
+        T = T or [torch.empty((p.shape[0], p.shape[0]), dtype=torch.float64, device=p.device)
+                  for p in psis]
+        arr = (_lib.oa_instance * max(count, 1))()
+        for i, p in enumerate(psis):
+            _need(p, f"psis[{i}]")
+            _need(outs[i], f"outs[{i}]", p.shape[0])
+            arr[i] = _lib.oa_instance(p.shape[0], p.shape[1] // 2, per(nkx0, i), per(nspin, i),
+                                      _ptr(p), _ld(p), _ptr(outs[i]), _ld(outs[i]),
+                                      _ptr(S[i]), _ptr(T[i]), per(tol, i), per(maxit, i), 0)
+        self.handle.use_stream()
+        check(_lib.lib().oa_batch_plan(self.b, count, arr, prec_code(prec)), "oa_batch_plan")
+        self._keep = [psis, outs, S, T]
+        self.psis, self.outs, self.S, self.T = psis, outs, S, T
+        self.count = count
+
+    def run(self) -> None:
+        self.handle.use_stream()
+        check(_lib.lib().oa_batch_run(self.b), "oa_batch_run")
+
+    def launch(self) -> None:
+        """Replay the step as a hipGraph (captured on first use after plan)."""
+        self.handle.use_stream()
+        check(_lib.lib().oa_batch_launch(self.b), "oa_batch_launch")
+
+    def reports(self):
+        """[(iterations, done, status, error)] per instance (synchronous)."""
+        arr = (_lib.oa_ns_report * max(self.count, 1))()
+        check(_lib.lib().oa_batch_reports(self.b, arr, self.count), "oa_batch_reports")
+        return [(r.iterations, r.done, r.status, r.error) for r in arr[:self.count]]
+
+    def tune(self, reps: int = 3) -> None:
+        self.handle.use_stream()
+        check(_lib.lib().oa_batch_tune(self.b, reps), "oa_batch_tune")
+
+    def config(self, i: int):
+        t, w = ctypes.c_int(0), ctypes.c_int(0)
+        check(_lib.lib().oa_batch_config(self.b, i, ctypes.byref(t), ctypes.byref(w)),
+              "oa_batch_config")
+        return t.value, w.value
+
+    def table(self) -> str:
+        L = _lib.lib()
+        need = L.oa_batch_get_table(self.b, None, 0)
+        buf = ctypes.create_string_buffer(max(need, 1))
+        check(L.oa_batch_get_table(self.b, buf, len(buf)), "oa_batch_get_table")
+        return buf.value.decode()
+
+    def set_table(self, text: str) -> None:
+        check(_lib.lib().oa_batch_set_table(self.b, text.encode()), "oa_batch_set_table")
+
+    def footprint(self) -> int:
+        return int(_lib.lib().oa_batch_footprint(self.b))
+
+    def close(self) -> None:
+        if getattr(self, "b", None):
+            _lib.lib().oa_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def instance_footprint(n: int, npw: int, prec="fp64") -> int:
+    """Device bytes one instance of (n, npw) holds inside a batch."""
+    return int(_lib.lib().oa_instance_footprint(n, npw, prec_code(prec)))
+
+
 _default: dict[int, Handle] = {}
 
 
