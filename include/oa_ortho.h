@@ -117,6 +117,23 @@ int oa_dgemm(oa_handle_t h, char transa, char transb, int m, int n, int k,
              double alpha, const double *a, int lda, const double *b, int ldb,
              double beta, double *c, int ldc);
 
+/* Replaces the reference's ZGEMM (src_mathlib/zgemm.f:1-411), the GEMM of
+ * the complex-math PairCalculator build (CP_PAIRCALC_USES_COMPLEX_MATH:
+ * ckPairCalculator.h:26-48 declares zgemm_; myGEMM promotes alpha and beta
+ * to complex, ckPairCalculator.C:56-65).  Column-major COMPLEX*16 (re, im
+ * pairs); alpha and beta point at (re, im).  op in {N, T, C}.  Host or device
+ * pointers, synchronous; invalid arguments return silently (xerbla.f). */
+void zgemm_(const char *transa, const char *transb, const int *m, const int *n,
+            const int *k, const double *alpha, const double *a, const int *lda,
+            const double *b, const int *ldb, const double *beta, double *c,
+            const int *ldc);
+
+/* Same operation on device pointers (alpha, beta in host memory),
+ * asynchronous on the handle's stream; returns the BLAS INFO value. */
+int oa_zgemm(oa_handle_t h, char transa, char transb, int m, int n, int k,
+             const double *alpha, const double *a, int lda, const double *b,
+             int ldb, const double *beta, double *c, int ldc);
+
 /* ---- path level (device pointers, asynchronous on the handle's stream) -- */
 
 /* Symmetric PairCalculator forward: S = alpha * sum_k w_k psi_x[k] psi_y[k]

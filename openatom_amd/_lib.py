@@ -109,6 +109,9 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_backmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
     _sig(L.oa_ortho_step, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp, _l,
          ctypes.POINTER(_i), ctypes.POINTER(_d))
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _sig(L.oa_zgemm, _i, _vp, ctypes.c_char, ctypes.c_char, _i, _i, _i, _dp, _vp, _i, _vp, _i,
+         _dp, _vp, _i)
     _sig(L.oa_batch_create, _i, _vp, ctypes.POINTER(_vp))
     _sig(L.oa_batch_destroy, _i, _vp)
     _sig(L.oa_batch_plan, _i, _vp, _i, ctypes.POINTER(oa_instance), _i)

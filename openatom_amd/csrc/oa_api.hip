@@ -236,6 +236,52 @@ int oa_dgemm(oa_handle_t c, char ta, char tb, int m, int n, int k, double alpha,
   return enqueue_dgemm(c->stream, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc);
 }
 
+// Run a column-major GEMM of `esz`-byte elements on host or device buffers:
+// device buffers go straight to `run`; host buffers (Charm++ message
+// payloads) are staged through the workspace, after `extra` bytes the GEMM
+// itself needs.  Synchronous.
+extern "C++" {
+template <class Run>
+int staged_gemm(oa_context *h, size_t esz, char ta, char tb, int m, int n, int k,
+                bool need_ab, bool beta0, const void *a, int lda, const void *b, int ldb,
+                void *c, int ldc, size_t extra, Run run) {
+  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+  const size_t na = need_ab ? span_elems(lda, nota ? m : k, nota ? k : m) : 0;
+  const size_t nb = need_ab ? span_elems(ldb, notb ? k : n, notb ? n : k) : 0;
+  const size_t nc = span_elems(ldc, m, n);
+  const bool dev = (!need_ab || (is_device_ptr(a) && is_device_ptr(b))) && is_device_ptr(c);
+  hipStream_t s = h->stream;
+  const size_t al = 256;
+  auto upb = [&](size_t e) { return (e * esz + al - 1) / al * al; };
+  int st;
+  if (dev) {
+    st = oa_ws_require(h, extra);
+    if (st == OA_SUCCESS) st = run(a, b, c, (char *)h->ws.ptr, h->ws.bytes);
+    if (st == OA_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = OA_ERR_HIP;
+    return st;
+  }
+  st = oa_ws_require(h, upb(na) + upb(nb) + upb(nc) + extra);
+  if (st) return st;
+  char *da = (char *)h->ws.ptr;
+  char *db = da + upb(na);
+  char *dc = db + upb(nb);
+  char *dx = dc + upb(nc);
+  hipError_t e = hipSuccess;
+  if (na) e = hipMemcpyAsync(da, a, na * esz, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && nb) e = hipMemcpyAsync(db, b, nb * esz, hipMemcpyHostToDevice, s);
+  // C is read when beta != 0; with beta == 0 and ldc > m the rows between m
+  // and ldc are not part of C and are carried through unchanged
+  if (e == hipSuccess && (!beta0 || ldc != m))
+    e = hipMemcpyAsync(dc, c, nc * esz, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) st = run(da, db, dc, dx, h->ws.bytes - (size_t)(dx - da));
+  else st = OA_ERR_HIP;
+  if (st == OA_SUCCESS && hipMemcpyAsync(c, dc, nc * esz, hipMemcpyDeviceToHost, s) != hipSuccess)
+    st = OA_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) st = OA_ERR_HIP;
+  return st;
+}
+}  // extern "C++"
+
 void dgemm_(const char *transa, const char *transb, const int *pm, const int *pn, const int *pk,
             const double *palpha, const double *a, const int *plda, const double *b,
             const int *pldb, const double *pbeta, double *c, const int *pldc) {
@@ -249,48 +295,54 @@ void dgemm_(const char *transa, const char *transb, const int *pm, const int *pn
     fprintf(stderr, "[oa_ortho] dgemm_: no usable gfx950 device; C left unchanged\n");
     return;
   }
-  const bool nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
-  const bool need_ab = (alpha != 0.0 && k != 0);
-  const size_t na = need_ab ? span_elems(lda, nota ? m : k, nota ? k : m) : 0;
-  const size_t nb = need_ab ? span_elems(ldb, notb ? k : n, notb ? n : k) : 0;
-  const size_t nc = span_elems(ldc, m, n);
-  const bool dev = (!need_ab || (is_device_ptr(a) && is_device_ptr(b))) && is_device_ptr(c);
-  hipStream_t s = h->stream;
-  int st;
-  if (dev) {
-    st = enqueue_dgemm(s, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-    if (st == OA_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = OA_ERR_HIP;
-  } else {
-    // Host buffers (Charm++ message payloads): stage through the workspace.
-    const size_t al = 32;
-    auto up = [&](size_t e) { return (e + al - 1) / al * al; };
-    st = oa_ws_require(h, (up(na) + up(nb) + up(nc)) * sizeof(double));
-    if (st == OA_SUCCESS) {
-      double *da = (double *)h->ws.ptr;
-      double *db = da + up(na);
-      double *dc = db + up(nb);
-      hipError_t e = hipSuccess;
-      if (na) e = hipMemcpyAsync(da, a, na * sizeof(double), hipMemcpyHostToDevice, s);
-      if (e == hipSuccess && nb)
-        e = hipMemcpyAsync(db, b, nb * sizeof(double), hipMemcpyHostToDevice, s);
-      if (e == hipSuccess && beta != 0.0)
-        e = hipMemcpyAsync(dc, c, nc * sizeof(double), hipMemcpyHostToDevice, s);
-      if (e == hipSuccess && beta == 0.0 && ldc != m) {
-        // rows between m and ldc are not part of C: carry them through unchanged
-        e = hipMemcpyAsync(dc, c, nc * sizeof(double), hipMemcpyHostToDevice, s);
-      }
-      if (e == hipSuccess)
-        st = enqueue_dgemm(s, ta, tb, m, n, k, alpha, da, lda, db, ldb, beta, dc, ldc);
-      else
-        st = OA_ERR_HIP;
-      if (st == OA_SUCCESS &&
-          hipMemcpyAsync(c, dc, nc * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
-        st = OA_ERR_HIP;
-      if (hipStreamSynchronize(s) != hipSuccess) st = OA_ERR_HIP;
-    }
+  const int st = staged_gemm(
+      h, sizeof(double), ta, tb, m, n, k, alpha != 0.0 && k != 0, beta == 0.0, a, lda, b, ldb,
+      c, ldc, 0,
+      [&](const void *da, const void *db, void *dc, char *, size_t) {
+        return enqueue_dgemm(h->stream, ta, tb, m, n, k, alpha, (const double *)da, lda,
+                             (const double *)db, ldb, beta, (double *)dc, ldc);
+      });
+  if (st != OA_SUCCESS) fprintf(stderr, "[oa_ortho] dgemm_ failed: %s\n", oa_status_string(st));
+}
+
+int oa_zgemm(oa_handle_t c, char ta, char tb, int m, int n, int k, const double *alpha,
+             const double *a, int lda, const double *b, int ldb, const double *beta,
+             double *cc, int ldc) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (!alpha || !beta) return !alpha ? 6 : 11;
+  const int info = dgemm_info(ta, tb, m, n, k, lda, ldb, ldc);  // zgemm.f:175-196, same codes
+  if (info) return info;
+  const bool a0 = alpha[0] == 0.0 && alpha[1] == 0.0, b1 = beta[0] == 1.0 && beta[1] == 0.0;
+  if (m == 0 || n == 0 || ((a0 || k == 0) && b1)) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  const int st = oa_ws_require(c, oa_zgemm_ws_bytes(ta, tb, m, n, k));
+  if (st) return st;
+  return oa_launch_zgemm(c->stream, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, cc, ldc,
+                         c->ws.ptr, c->ws.bytes);
+}
+
+void zgemm_(const char *transa, const char *transb, const int *pm, const int *pn, const int *pk,
+            const double *alpha, const double *a, const int *plda, const double *b,
+            const int *pldb, const double *beta, double *c, const int *pldc) {
+  const char ta = *transa, tb = *transb;
+  const int m = *pm, n = *pn, k = *pk, lda = *plda, ldb = *pldb, ldc = *pldc;
+  if (dgemm_info(ta, tb, m, n, k, lda, ldb, ldc)) return;  // XERBLA is silent (xerbla.f)
+  const bool a0 = alpha[0] == 0.0 && alpha[1] == 0.0;
+  const bool b0 = beta[0] == 0.0 && beta[1] == 0.0, b1 = beta[0] == 1.0 && beta[1] == 0.0;
+  if (m == 0 || n == 0 || ((a0 || k == 0) && b1)) return;
+  oa_context *h = default_handle();
+  if (!h) {
+    fprintf(stderr, "[oa_ortho] zgemm_: no usable gfx950 device; C left unchanged\n");
+    return;
   }
-  if (st != OA_SUCCESS)
-    fprintf(stderr, "[oa_ortho] dgemm_ failed: %s\n", oa_status_string(st));
+  const int st = staged_gemm(
+      h, 2 * sizeof(double), ta, tb, m, n, k, !a0 && k != 0, b0, a, lda, b, ldb, c, ldc,
+      oa_zgemm_ws_bytes(ta, tb, m, n, k),
+      [&](const void *da, const void *db, void *dc, char *wx, size_t wbytes) {
+        return oa_launch_zgemm(h->stream, ta, tb, m, n, k, alpha, (const double *)da, lda,
+                               (const double *)db, ldb, beta, (double *)dc, ldc, wx, wbytes);
+      });
+  if (st != OA_SUCCESS) fprintf(stderr, "[oa_ortho] zgemm_ failed: %s\n", oa_status_string(st));
 }
 
 // ------------------------------------------------------------- path level

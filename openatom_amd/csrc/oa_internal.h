@@ -87,6 +87,14 @@ int oa_launch_s_to_t(hipStream_t s, int n, const double *S, int nspin,
                      double tol, int maxit, double *T, oa_ns_report *report,
                      void *ws);
 
+// ---------------------------------------------------------------- complex
+// Column-major ZGEMM on device buffers (COMPLEX*16 interleaved); alpha and
+// beta are (re, im) pairs in host memory.  Workspace: oa_zgemm_ws_bytes.
+size_t oa_zgemm_ws_bytes(char ta, char tb, int m, int n, int k);
+int oa_launch_zgemm(hipStream_t s, char ta, char tb, int m, int n, int k, const double *alpha,
+                    const double *a, int lda, const double *b, int ldb, const double *beta,
+                    double *c, int ldc, void *ws, size_t ws_bytes);
+
 // ---------------------------------------------------------------- split
 size_t oa_split_ws_bytes(int prec, int n, int K);
 int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K,

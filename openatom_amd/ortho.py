@@ -88,6 +88,16 @@ class Handle:
         return _lib.lib().oa_dgemm(self.h, transa.encode(), transb.encode(), m, n, k, alpha,
                                    _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc)
 
+    def zgemm(self, transa: str, transb: str, m: int, n: int, k: int, alpha: complex,
+              a: torch.Tensor, lda: int, b: torch.Tensor, ldb: int, beta: complex,
+              c: torch.Tensor, ldc: int) -> int:
+        """Column-major ZGEMM on complex128 device buffers (returns the BLAS
+        INFO); the complex-math PairCalculator's GEMM."""
+        al = (ctypes.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+        be = (ctypes.c_double * 2)(complex(beta).real, complex(beta).imag)
+        return _lib.lib().oa_zgemm(self.h, transa.encode(), transb.encode(), m, n, k, al,
+                                   _ptr(a), lda, _ptr(b), ldb, be, _ptr(c), ldc)
+
     def overlap_sym(self, psi: torch.Tensor, nkx0: int = 1, alpha: float = 2.0,
                     prec="fp64", S: torch.Tensor | None = None, diag: float = 0.0,
                     maxdev: torch.Tensor | None = None) -> torch.Tensor:

@@ -84,6 +84,21 @@ def make_states(n: int, npw: int, eps: float = 1e-3, seed: int = 0, nkx0: int = 
     return Q.contiguous()
 
 
+def perturb_(psi: torch.Tensor, eps: float, seed: int, nkx0: int = 1) -> torch.Tensor:
+    """One simulated MD step on a resident block, in place: psi += eps times
+    an envelope-shaped complex normal block (each row of metric norm eps),
+    drawn with torch's generator seeded by `seed` on psi's device."""
+    n, K = psi.shape
+    npw = K // 2
+    gen = torch.Generator(device=psi.device)
+    gen.manual_seed(seed)
+    P = _complex_normal_block(gen, n, npw, psi.device)
+    w = torch.from_numpy(metric_weights(npw, nkx0)).to(psi.device)
+    nrm = torch.sqrt(((P * w) * P).sum(dim=1, keepdim=True))
+    psi += P * (eps / nrm)
+    return psi
+
+
 def make_force(n: int, npw: int, std: float = 0.1, seed: int = 1,
                device: str | torch.device = "cpu", rng: str = "numpy") -> torch.Tensor:
     """Force block F for the Lambda product: i.i.d. complex normal, std 0.1."""

@@ -38,6 +38,12 @@
  *                        backward, kx0 rescale (CP_State_GSpacePlane.C:3278-3281).
  *   oao_lambda           asymmetric forward, alpha = 1 (cpaimd.C:2510-2516,
  *                        ckPairCalculator.C:889): L[x][y] = sum_k psi_x[k] F_y[k].
+ *   oao_zgemm            src_mathlib/zgemm.f:160-411 (netlib reference ZGEMM,
+ *                        the complex-math PairCalculator's GEMM,
+ *                        ckPairCalculator.h:26-48): argument checks, quick
+ *                        returns and the nine loop nests of op(A) x op(B) in
+ *                        {N, T, C}^2 with their summation order.  Pinned
+ *                        bitwise against the compiled reference zgemm_.
  *
  * Every matrix of states is row-major [n][K] doubles with K = 2*npw (complex
  * coefficients interleaved re,im), exactly the collated layout the reference
@@ -163,6 +169,105 @@ void oao_dgemm(const char *ta, const char *tb, const int *m, const int *n,
 }
 
 static oao_dgemm_fn pick(oao_dgemm_fn f) { return f ? f : oao_dgemm; }
+
+/* ---- complex (COMPLEX*16 = two doubles, re then im) ------------------- */
+typedef struct { double re, im; } oz;
+static oz zmul(oz a, oz b) {
+  oz r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return r;
+}
+static oz zadd(oz a, oz b) {
+  oz r = {a.re + b.re, a.im + b.im};
+  return r;
+}
+static oz zconj(oz a) {
+  oz r = {a.re, -a.im};
+  return r;
+}
+static int zis0(oz a) { return a.re == 0.0 && a.im == 0.0; }
+static int zis1(oz a) { return a.re == 1.0 && a.im == 0.0; }
+#define ZX(M, i, j, ld) (M)[(size_t)(i) + (size_t)(j) * (size_t)(ld)]
+
+/* Restatement of zgemm.f:160-411; A, B, C are column-major COMPLEX*16. */
+int oao_zgemm_info(char ta, char tb, int m, int n, int k, const double *palpha,
+                   const double *pA, int lda, const double *pB, int ldb,
+                   const double *pbeta, double *pC, int ldc) {
+  const oz *A = (const oz *)pA, *B = (const oz *)pB;
+  oz *C = (oz *)pC;
+  const oz alpha = {palpha[0], palpha[1]}, beta = {pbeta[0], pbeta[1]};
+  const oz zero = {0.0, 0.0};
+  int nota = lsame_c(ta, 'N'), notb = lsame_c(tb, 'N');
+  int conja = lsame_c(ta, 'C'), conjb = lsame_c(tb, 'C');
+  int nrowa = nota ? m : k;
+  int nrowb = notb ? k : n;
+  int info = 0;
+  if (!nota && !conja && !lsame_c(ta, 'T')) info = 1;
+  else if (!notb && !conjb && !lsame_c(tb, 'T')) info = 2;
+  else if (m < 0) info = 3;
+  else if (n < 0) info = 4;
+  else if (k < 0) info = 5;
+  else if (lda < (nrowa > 1 ? nrowa : 1)) info = 8;
+  else if (ldb < (nrowb > 1 ? nrowb : 1)) info = 10;
+  else if (ldc < (m > 1 ? m : 1)) info = 13;
+  if (info != 0) return info;
+  if (m == 0 || n == 0 || ((zis0(alpha) || k == 0) && zis1(beta))) return 0;
+  if (zis0(alpha)) {
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < m; i++)
+        ZX(C, i, j, ldc) = zis0(beta) ? zero : zmul(beta, ZX(C, i, j, ldc));
+    return 0;
+  }
+  if (notb && nota) { /* C := alpha*A*B + beta*C  (zgemm.f:232-249) */
+    for (int j = 0; j < n; j++) {
+      if (zis0(beta)) {
+        for (int i = 0; i < m; i++) ZX(C, i, j, ldc) = zero;
+      } else if (!zis1(beta)) {
+        for (int i = 0; i < m; i++) ZX(C, i, j, ldc) = zmul(beta, ZX(C, i, j, ldc));
+      }
+      for (int l = 0; l < k; l++) {
+        if (!zis0(ZX(B, l, j, ldb))) {
+          oz temp = zmul(alpha, ZX(B, l, j, ldb));
+          for (int i = 0; i < m; i++)
+            ZX(C, i, j, ldc) = zadd(ZX(C, i, j, ldc), zmul(temp, ZX(A, i, l, lda)));
+        }
+      }
+    }
+    return 0;
+  }
+  if (nota) { /* C := alpha*A*op(B) + beta*C, op = conjg(B') or B' (zgemm.f:288-335) */
+    for (int j = 0; j < n; j++) {
+      if (zis0(beta)) {
+        for (int i = 0; i < m; i++) ZX(C, i, j, ldc) = zero;
+      } else if (!zis1(beta)) {
+        for (int i = 0; i < m; i++) ZX(C, i, j, ldc) = zmul(beta, ZX(C, i, j, ldc));
+      }
+      for (int l = 0; l < k; l++) {
+        if (!zis0(ZX(B, j, l, ldb))) {
+          oz bjl = conjb ? zconj(ZX(B, j, l, ldb)) : ZX(B, j, l, ldb);
+          oz temp = zmul(alpha, bjl);
+          for (int i = 0; i < m; i++)
+            ZX(C, i, j, ldc) = zadd(ZX(C, i, j, ldc), zmul(temp, ZX(A, i, l, lda)));
+        }
+      }
+    }
+    return 0;
+  }
+  /* op(A) = A' or conjg(A'): dot products (zgemm.f:250-285, 336-405) */
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) {
+      oz temp = zero;
+      for (int l = 0; l < k; l++) {
+        oz a = conja ? zconj(ZX(A, l, i, lda)) : ZX(A, l, i, lda);
+        oz bv;
+        if (notb) bv = ZX(B, l, j, ldb);
+        else bv = conjb ? zconj(ZX(B, j, l, ldb)) : ZX(B, j, l, ldb);
+        temp = zadd(temp, zmul(a, bv));
+      }
+      if (zis0(beta)) ZX(C, i, j, ldc) = zmul(alpha, temp);
+      else ZX(C, i, j, ldc) = zadd(zmul(alpha, temp), zmul(beta, ZX(C, i, j, ldc)));
+    }
+  return 0;
+}
 
 /* ckPairCalculator.C:846-889: out (row-major [nX][nY]) = alpha * left * right^T
  * over K doubles per row. */

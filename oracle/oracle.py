@@ -80,6 +80,10 @@ def lib() -> ctypes.CDLL:
                                        _dp, ctypes.c_void_p]
         L.oao_max_threads.restype = ctypes.c_int
         L.oao_set_threads.argtypes = [ctypes.c_int]
+        L.oao_zgemm_info.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
+                                     ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.oao_zgemm_info.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -129,6 +133,29 @@ def dgemm_cm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, which="oracle
       ctypes.byref(ci(m)), ctypes.byref(ci(n)), ctypes.byref(ci(k)),
       ctypes.byref(cd(alpha)), _p(A), ctypes.byref(ci(lda)), _p(B),
       ctypes.byref(ci(ldb)), ctypes.byref(cd(beta)), _p(C), ctypes.byref(ci(ldc)))
+    return 0
+
+
+def _pz(a: np.ndarray):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def zgemm_cm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, which="oracle"):
+    """Column-major ZGEMM on flat complex128 buffers through either the
+    restatement or the reference's compiled zgemm_ (zgemm.f)."""
+    al = np.array([complex(alpha)], dtype=np.complex128)
+    be = np.array([complex(beta)], dtype=np.complex128)
+    if which == "oracle":
+        return lib().oao_zgemm_info(ta.encode(), tb.encode(), m, n, k, _pz(al), _pz(A), lda,
+                                    _pz(B), ldb, _pz(be), _pz(C), ldc)
+    f = ref().zgemm_
+    f.restype = None
+    ci = ctypes.c_int
+    f(ctypes.c_char_p(ta.encode()), ctypes.c_char_p(tb.encode()),
+      ctypes.byref(ci(m)), ctypes.byref(ci(n)), ctypes.byref(ci(k)),
+      _pz(al), _pz(A), ctypes.byref(ci(lda)), _pz(B), ctypes.byref(ci(ldb)), _pz(be), _pz(C),
+      ctypes.byref(ci(ldc)))
     return 0
 
 

@@ -169,7 +169,7 @@ def test_backmul_vs_oracle(handle, oracle, transM):
     assert rel(got.cpu().numpy(), want) < 1e-12
 
 
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))))
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "step_*.npz"))))
 def test_golden_fixtures_on_gpu(handle, path):
     g = np.load(path)
     psi = torch.from_numpy(g["psi"]).to(DEV)
