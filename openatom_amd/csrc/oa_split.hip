@@ -13,8 +13,9 @@
 // Products.  sum_k a_p[i,k] b_q[j,k] over one chunk is accumulated exactly:
 // int8 digits on v_mfma_i32_32x32x32_i8 into int32 (bounded by the chunk
 // length, which the host caps), fp16 / bf16 digits on
-// v_mfma_f32_32x32x16_{f16,bf16} into fp32 promoted to FP64 every 128 k
-// (sums stay below 2^24 and are therefore exact).  Products are grouped by
+// v_mfma_f32_32x32x16_{f16,bf16} and fp32 digits on v_mfma_f32_32x32x2_f32
+// into fp32 promoted to FP64 every 128 k (sums stay below 2^24 and are
+// therefore exact).  Products are grouped by
 // level L = p + q <= S + 1 (the Ozaki triangular truncation); level sums are
 // combined in FP64 with weights R^(-(L-2)) and the chunk scales.  Because the
 // level sums are exact, every result is independent of the tile shape, the
@@ -76,8 +77,8 @@ struct TileCfg<64> {
   static constexpr int WJ = 1;
 };
 
-// Digit element type: int8 or 16-bit float.  ES = bytes per element.
-enum DType { DT_I8 = 0, DT_F16 = 1, DT_BF16 = 2 };
+// Digit element type: int8, 16-bit float or fp32.  ES = bytes per element.
+enum DType { DT_I8 = 0, DT_F16 = 1, DT_BF16 = 2, DT_F32 = 3 };
 
 template <int DT>
 struct DTraits;
@@ -99,12 +100,19 @@ struct DTraits<DT_BF16> {
   static constexpr int ES = 2;
   typedef f16v acc_t;
 };
+template <>
+struct DTraits<DT_F32> {
+  typedef float elem;
+  static constexpr int ES = 4;
+  typedef f16v acc_t;
+};
 
 template <int DT>
 __device__ __forceinline__ typename DTraits<DT>::elem to_digit(double a) {
   if constexpr (DT == DT_I8) return (int8_t)(int)a;
   else if constexpr (DT == DT_F16) return (_Float16)(float)a;
-  else return (__bf16)(float)a;
+  else if constexpr (DT == DT_BF16) return (__bf16)(float)a;
+  else return (float)a;
 }
 
 template <int DT>
@@ -115,9 +123,19 @@ __device__ __forceinline__ typename DTraits<DT>::acc_t mfma_digit(i4v a, i4v b,
   } else if constexpr (DT == DT_F16) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8v, a),
                                                   __builtin_bit_cast(h8v, b), c, 0, 0, 0);
-  } else {
+  } else if constexpr (DT == DT_BF16) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(b8v, a),
                                                    __builtin_bit_cast(b8v, b), c, 0, 0, 0);
+  } else {
+    // four k = 2 steps: lanes < 32 hold contraction 4h+e of their row, lanes
+    // >= 32 the other half, the same pairing for both operands
+    const float fa[4] = {__int_as_float(a.x), __int_as_float(a.y), __int_as_float(a.z),
+                         __int_as_float(a.w)};
+    const float fb[4] = {__int_as_float(b.x), __int_as_float(b.y), __int_as_float(b.z),
+                         __int_as_float(b.w)};
+#pragma unroll
+    for (int e = 0; e < 4; e++) c = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[e], fb[e], c, 0, 0, 0);
+    return c;
   }
 }
 
@@ -264,9 +282,9 @@ template <int DT, int S>
 __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
   typedef typename DTraits<DT>::elem E;
   constexpr int ES = DTraits<DT>::ES;
-  constexpr int PITCH = SLT + 16 / ES;  // LDS row pitch in elements (16-byte aligned rows)
-  constexpr int PER = 16 / ES;
-  __shared__ __attribute__((aligned(16))) E lt[S][SLT * PITCH];
+  constexpr int PITCH = SLT + 16;  // LDS row pitch in bytes (16-byte aligned rows)
+  // digits are integers |a| <= 127 in every type: the transpose holds int8
+  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * PITCH];
   E *DS = reinterpret_cast<E *>(a.DS);
   E *DB = reinterpret_cast<E *>(a.DB);
   const int nkt = a.Kp / SLT;
@@ -306,7 +324,7 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
         }
       }
 #pragma unroll
-      for (int p = 0; p < S; p++) lt[p][(2 * cp + e) * PITCH + ii] = to_digit<DT>(dB[e][p]);
+      for (int p = 0; p < S; p++) lt[p][(2 * cp + e) * PITCH + ii] = (int8_t)(int)dB[e][p];
     }
     if (a.want_ds) {
 #pragma unroll
@@ -316,11 +334,13 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
           const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
                                                     (((unsigned)(int)dS[1][p] & 0xffu) << 8));
           *reinterpret_cast<unsigned short *>(dst) = u;
-        } else {
+        } else if constexpr (ES == 2) {
           const E lo = to_digit<DT>(dS[0][p]), hi = to_digit<DT>(dS[1][p]);
           *reinterpret_cast<unsigned *>(dst) =
               (unsigned)__builtin_bit_cast(unsigned short, lo) |
               ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+        } else {
+          *reinterpret_cast<float2 *>(dst) = float2{(float)dS[0][p], (float)dS[1][p]};
         }
       }
     }
@@ -330,11 +350,21 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
   const int kk = t / 4, ib = (t % 4) * 16;
 #pragma unroll
   for (int p = 0; p < S; p++) {
-    E *dst = DB + p * sdb + kb_off<DT>(k0 + kk, i0 + ib, a.Kp);
+    E *dst = DB + p * sdb;
+    const i4v src = *reinterpret_cast<const i4v *>(&lt[p][kk * PITCH + ib]);
+    if constexpr (ES == 1) {
+      *reinterpret_cast<i4v *>(dst + kb_off<DT>(k0 + kk, i0 + ib, a.Kp)) = src;
+    } else {
+      E v[16];
 #pragma unroll
-    for (int c = 0; c < 16; c += PER)
-      *reinterpret_cast<i4v *>(dst + c) =
-          *reinterpret_cast<const i4v *>(&lt[p][kk * PITCH + ib + c]);
+      for (int c = 0; c < 16; c++)
+        v[c] = to_digit<DT>((double)(int8_t)((src[c / 4] >> (8 * (c % 4))) & 0xff));
+      // 16 bytes per store; a store never crosses a 32-byte k-block
+#pragma unroll
+      for (int c = 0; c < 16; c += 16 / ES)
+        *reinterpret_cast<i4v *>(dst + kb_off<DT>(k0 + kk, i0 + ib + c, a.Kp)) =
+            *reinterpret_cast<const i4v *>(&v[c]);
+    }
   }
   __syncthreads();
 }
@@ -462,6 +492,12 @@ __device__ __forceinline__ void load_digit_quad(const void *base, size_t off, do
     const int w = *reinterpret_cast<const int *>(reinterpret_cast<const int8_t *>(base) + off);
 #pragma unroll
     for (int r = 0; r < 4; r++) q[r] = (double)(int8_t)((w >> (8 * r)) & 0xff);
+  } else if constexpr (DT == DT_F32) {
+    const float4 w = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(base) + off);
+    q[0] = w.x;
+    q[1] = w.y;
+    q[2] = w.z;
+    q[3] = w.w;
   } else {
     typedef typename DTraits<DT>::elem E;
     const E *e = reinterpret_cast<const E *>(base) + off;
@@ -480,7 +516,9 @@ struct DigitCfg {
   static constexpr int NL = (MODE == 0) ? S : NLB;  // slices in the product and levels kept
   static constexpr int PANEL = TILE * 32;           // bytes of one slice panel per stage
   static constexpr int STAGE = 2 * NL * PANEL;      // X and Y slices of one stage
-  static constexpr int NBUF = 3;                    // LDS ring depth
+  // LDS ring depth: four stages (three in flight) where they fit the 160 KB
+  // of a workgroup, else three
+  static constexpr int NBUF = (4 * STAGE <= 160 * 1024) ? 4 : 3;
   static constexpr int SMEM = NBUF * STAGE;
   static constexpr int NW = TileCfg<TILE>::NW;
   static constexpr int NTHR = 64 * NW;
@@ -493,8 +531,8 @@ struct DigitCfg {
 //           + 2^(f_m + e(n) - 14) C) + beta out, where C uses NLB slices of
 //           X and of psi (levels < NLB) and psi~ is psi rebuilt from all S
 //           digits of DB.
-// The X and Y panels of three stages rotate through LDS, filled by
-// direct-to-LDS loads (global_load_lds_dwordx4) two stages ahead.
+// The X and Y panels of NBUF stages rotate through LDS, filled by
+// direct-to-LDS loads (global_load_lds_dwordx4) NBUF-1 stages ahead.
 template <int DT, int S, int MODE, int NLB, int TILE>
 __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, char *smem) {
   typedef typename DTraits<DT>::acc_t ACC;
@@ -565,13 +603,17 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
     }
   };
   const int nst = (kb - ka) / SK;
-  if (nst > 0) issue(0);
-  if (nst > 1) issue(1);
+#pragma unroll
+  for (int st = 0; st < NBUF - 1; st++)
+    if (st < nst) issue(st);
   for (int s = 0; s < nst; s++) {
-    if (s + 1 < nst) wait_vm<PERW>();
+    // stages s+1 .. min(nst-1, s+NBUF-2) may stay in flight
+    const int ahead = min(nst - 1 - s, NBUF - 2);
+    if (NBUF == 4 && ahead >= 2) wait_vm<2 * PERW>();
+    else if (ahead >= 1) wait_vm<PERW>();
     else wait_vm<0>();
     __syncthreads();
-    if (s + 2 < nst) issue(s + 2);
+    if (s + NBUF - 1 < nst) issue(s + NBUF - 1);
     const char *cur = smem + (s % NBUF) * STAGE;
     digit_stage<DT, NL, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
     if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
@@ -721,6 +763,7 @@ bool decode_prec(int prec, int &dt, int &S) {
   if (base == OA_PREC_INT8) dt = DT_I8;
   else if (base == OA_PREC_FP16) dt = DT_F16;
   else if (base == OA_PREC_BF16) dt = DT_BF16;
+  else if (base == OA_PREC_FP32) dt = DT_F32;
   else return false;
   return true;
 }
@@ -763,7 +806,7 @@ oa_split_plan make_plan(int prec, int n, int K, int tile) {
   oa_split_plan pl{};
   decode_prec(prec, pl.dt, pl.S);
   pl.tile = tile;
-  pl.es = (pl.dt == DT_I8) ? 1 : 2;
+  pl.es = (pl.dt == DT_I8) ? 1 : (pl.dt == DT_F32 ? 4 : 2);
   pl.np = cdv(n, tile) * tile;
   const int kq = tile > SLT ? tile : SLT;
   pl.Kp = cdv(K, kq) * kq;
@@ -1051,6 +1094,7 @@ int oa_split_grouped(hipStream_t s, int prec, int tile, int phase, const oa_grou
   const bool lean = (prec & OA_PREC_LEAN_BACKMUL) != 0;
   if (dt == DT_I8) return grouped_dt<DT_I8>(s, S, lean, tile, phase, g);
   if (dt == DT_F16) return grouped_dt<DT_F16>(s, S, lean, tile, phase, g);
+  if (dt == DT_F32) return grouped_dt<DT_F32>(s, S, lean, tile, phase, g);
   return grouped_dt<DT_BF16>(s, S, lean, tile, phase, g);
 }
 
@@ -1077,6 +1121,7 @@ int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K, const double 
   char *w = (char *)ws;
   if (dt == DT_I8) return dispatch_overlap<DT_I8>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
   if (dt == DT_F16) return dispatch_overlap<DT_F16>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
+  if (dt == DT_F32) return dispatch_overlap<DT_F32>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
   return dispatch_overlap<DT_BF16>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
 }
 
@@ -1095,6 +1140,9 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double 
                                    alpha, beta, out, ldout, w);
   if (dt == DT_F16)
     return dispatch_backmul<DT_F16>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
+                                    alpha, beta, out, ldout, w);
+  if (dt == DT_F32)
+    return dispatch_backmul<DT_F32>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
                                     alpha, beta, out, ldout, w);
   return dispatch_backmul<DT_BF16>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
                                    alpha, beta, out, ldout, w);

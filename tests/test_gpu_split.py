@@ -33,7 +33,7 @@ def block():
 
 
 @pytest.mark.parametrize("prec", ["int8x3", "int8x4", "int8x5", "int8x6", "fp16x3", "bf16x3",
-                                  "fp16x5", "bf16x5"])
+                                  "fp16x5", "bf16x5", "fp32x5"])
 def test_split_overlap_accuracy(handle, block, prec):
     S64 = handle.overlap_sym(block, nkx0=1, alpha=2.0, prec="fp64")
     Ss = handle.overlap_sym(block, nkx0=1, alpha=2.0, prec=prec)
@@ -54,7 +54,16 @@ def test_split_digit_types_agree(handle, block, slices):
     assert rel(b, a) < 1e-14 and rel(c, a) < 1e-14
 
 
-@pytest.mark.parametrize("prec", ["int8x4", "int8x5", "int8x6l", "fp16x5", "bf16x4"])
+@pytest.mark.parametrize("slices", [3, 5])
+def test_split_fp32_digits_agree(handle, block, slices):
+    """fp32 digits on v_mfma_f32_32x32x2_f32: the same exact level sums."""
+    a = handle.overlap_sym(block, nkx0=1, prec=f"int8x{slices}")
+    d = handle.overlap_sym(block, nkx0=1, prec=f"fp32x{slices}")
+    torch.cuda.synchronize()
+    assert rel(d, a) < 1e-14
+
+
+@pytest.mark.parametrize("prec", ["int8x4", "int8x5", "int8x6l", "fp16x5", "bf16x4", "fp32x5"])
 def test_split_backmul_accuracy(handle, block, prec):
     n = block.shape[0]
     S = handle.overlap_sym(block, nkx0=1, prec="fp64")
@@ -101,7 +110,7 @@ def test_split_ragged_sizes(handle, n, npw, nkx0):
     assert rel(handle.backmul(M, psi, prec="int8x5"), handle.backmul(M, psi, prec="fp64")) < 1e-6
 
 
-@pytest.mark.parametrize("prec", ["int8x5", "fp16x5", "int8x6l"])
+@pytest.mark.parametrize("prec", ["int8x5", "fp16x5", "fp32x5", "int8x6l"])
 def test_split_step_meets_residual_target(handle, prec):
     """n=512, npw=65536 (the plain-vs-split configuration): one split step
     leaves the orthonormality residual <= 1e-10 and T within 1e-10 of FP64."""
