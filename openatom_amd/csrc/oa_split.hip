@@ -41,7 +41,9 @@
 //   digit_nt      C[m][n] = sum_k X[m][k] Y[n][k] over digit slices:
 //                 MODE 0 = split-K partial tiles of S (upper triangle),
 //                 MODE 1 = back-multiply epilogue out = a*T*psi + b*out.
-//                 TILE 128: 512 threads, 8 waves as 4 x 2, wave tile 32 x 64;
+//                 TILE 128: 4 waves as 2 x 2 with 64 x 64 wave tiles (int8,
+//                 up to 5 levels; one wave per SIMD, up to 512 registers), else
+//                 8 waves as 4 x 2 with 32 x 64 wave tiles;
 //                 TILE 64: 256 threads, 4 waves as 2 x 2, wave tile 32 x 32.
 //   split_reduce  fixed-order sum of the partial tiles, FP64 G=0 term,
 //                 mirror, max|S - d I|.
@@ -64,17 +66,33 @@ constexpr long KCHUNK_TARGET = 8192;  // columns per scale block / split-K chunk
 
 __host__ __device__ inline int cdv(int a, int b) { return (a + b - 1) / b; }
 
-template <int TILE>
+// A workgroup tile of TILE x TILE outputs is split over NW waves, each
+// owning WI x WJ blocks of 32 x 32 (one MFMA accumulator per block and
+// level); WPE waves per SIMD are allowed, so a wave may hold 512 / WPE
+// registers.  WIDE (64 x 64 wave tiles, one wave per SIMD) reads a third
+// fewer LDS bytes per MFMA; it is used where the accumulators fit.
+template <int TILE, bool WIDE>
 struct TileCfg;
 template <>
-struct TileCfg<128> {
-  static constexpr int NW = 8;  // waves: 4 rows x 2 columns
+struct TileCfg<128, true> {
+  static constexpr int NW = 4;  // waves: 2 rows x 2 columns, 64 x 64 each
+  static constexpr int WI = 2;  // 32-row MFMA blocks per wave
   static constexpr int WJ = 2;  // 32-column MFMA blocks per wave
+  static constexpr int WPE = 1;
 };
 template <>
-struct TileCfg<64> {
-  static constexpr int NW = 4;  // waves: 2 rows x 2 columns
+struct TileCfg<128, false> {
+  static constexpr int NW = 8;  // waves: 4 rows x 2 columns, 32 x 64 each
+  static constexpr int WI = 1;
+  static constexpr int WJ = 2;
+  static constexpr int WPE = 2;
+};
+template <bool WIDE>
+struct TileCfg<64, WIDE> {
+  static constexpr int NW = 4;  // waves: 2 rows x 2 columns, 32 x 32 each
+  static constexpr int WI = 1;
   static constexpr int WJ = 1;
+  static constexpr int WPE = 2;
 };
 
 // Digit element type: int8, 16-bit float or fp32.  ES = bytes per element.
@@ -450,21 +468,24 @@ __device__ __forceinline__ void upper_tile_d(int tile, int T, int &bi, int &bj) 
   bj = b + rem;
 }
 
-// Level-grouped digit products of one 32-byte stage for a 32 x (32*WJ) wave
-// tile: acc[l] += sum over (p, q) with p + q = l < NL of X_p Y_q^T.  Panels
-// hold TILE rows x 32 bytes per slice with the two 16-byte halves of row r
-// swapped when bit 3 of r is set, which spreads any 16 consecutive lanes'
-// 16-byte fragment reads over all 64 LDS banks.
-template <int DT, int NL, int WJ, int TILE>
+// Level-grouped digit products of one 32-byte stage for a (32*WI) x (32*WJ)
+// wave tile: acc[l] += sum over (p, q) with p + q = l < NL of X_p Y_q^T.
+// Panels hold TILE rows x 32 bytes per slice with the two 16-byte halves of
+// row r swapped when bit 3 of r is set, which spreads any 16 consecutive
+// lanes' 16-byte fragment reads over all 64 LDS banks.
+template <int DT, int NL, int WI, int WJ, int TILE>
 __device__ __forceinline__ void digit_stage(const char *Xs, const char *Ys, int xrow, int yrow,
                                             int lane,
-                                            typename DTraits<DT>::acc_t (&acc)[NL][WJ]) {
+                                            typename DTraits<DT>::acc_t (&acc)[NL][WI][WJ]) {
   constexpr int PANEL = TILE * 32;
   const int r = lane & 31;
   const int boff = r * 32 + 16 * ((lane >> 5) ^ ((r >> 3) & 1));
-  i4v a[NL];
+  i4v a[NL][WI];
 #pragma unroll
-  for (int p = 0; p < NL; p++) a[p] = *reinterpret_cast<const i4v *>(Xs + p * PANEL + xrow * 32 + boff);
+  for (int p = 0; p < NL; p++)
+#pragma unroll
+    for (int i = 0; i < WI; i++)
+      a[p][i] = *reinterpret_cast<const i4v *>(Xs + p * PANEL + (xrow + i * 32) * 32 + boff);
 #pragma unroll
   for (int q = 0; q < NL; q++) {
     i4v b[WJ];
@@ -474,7 +495,10 @@ __device__ __forceinline__ void digit_stage(const char *Xs, const char *Ys, int 
 #pragma unroll
     for (int p = 0; p + q < NL; p++)
 #pragma unroll
-      for (int j = 0; j < WJ; j++) acc[p + q][j] = mfma_digit<DT>(a[p], b[j], acc[p + q][j]);
+      for (int i = 0; i < WI; i++)
+#pragma unroll
+        for (int j = 0; j < WJ; j++)
+          acc[p + q][i][j] = mfma_digit<DT>(a[p][i], b[j], acc[p + q][i][j]);
   }
 }
 
@@ -514,13 +538,18 @@ __device__ __forceinline__ void load_digit_quad(const void *base, size_t off, do
 template <int DT, int S, int MODE, int NLB, int TILE>
 struct DigitCfg {
   static constexpr int NL = (MODE == 0) ? S : NLB;  // slices in the product and levels kept
+  // int32 accumulators of up to 5 levels fit a 64 x 64 wave tile; FP32 ones
+  // carry an FP64 promotion copy and stay on 32 x 64
+  static constexpr bool WIDE = (TILE == 128 && DT == DT_I8 && NL <= 5);
+  typedef TileCfg<TILE, WIDE> T;
+  static constexpr int WI = T::WI, WJ = T::WJ, WPE = T::WPE;
   static constexpr int PANEL = TILE * 32;           // bytes of one slice panel per stage
   static constexpr int STAGE = 2 * NL * PANEL;      // X and Y slices of one stage
   // LDS ring depth: four stages (three in flight) where they fit the 160 KB
   // of a workgroup, else three
   static constexpr int NBUF = (4 * STAGE <= 160 * 1024) ? 4 : 3;
   static constexpr int SMEM = NBUF * STAGE;
-  static constexpr int NW = TileCfg<TILE>::NW;
+  static constexpr int NW = T::NW;
   static constexpr int NTHR = 64 * NW;
 };
 
@@ -544,7 +573,10 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   constexpr int STAGE = C::STAGE;
   constexpr int NBUF = C::NBUF;
   constexpr int NW = C::NW;
-  constexpr int WJ = TileCfg<TILE>::WJ;
+  constexpr int WI = C::WI;
+  constexpr int WJ = C::WJ;
+  constexpr int WC = TILE / (32 * WJ);    // waves per tile row
+  static_assert(NW == (TILE / (32 * WI)) * WC, "waves must cover the tile");
   constexpr int NQ = TILE / 32;           // 1 KB wave-loads per slice panel
   constexpr int NI = 2 * NL * NQ;         // wave-loads per stage
   constexpr int PERW = NI / NW;           // per wave
@@ -568,18 +600,22 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   const int ka = (MODE == 0) ? z * a.kc : 0;
   const int kb = (MODE == 0) ? min(a.kfull, ka + a.kc) : a.kfull;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int xrow = (w >> 1) * 32, yrow = (w & 1) * 32 * WJ;
+  const int xrow = (w / WC) * 32 * WI, yrow = (w % WC) * 32 * WJ;
 
-  ACC acc[NL][WJ];
-  double facc[WJ][16];
+  ACC acc[NL][WI][WJ];
+  double facc[WI][WJ][16];
 #pragma unroll
   for (int l = 0; l < NL; l++)
 #pragma unroll
-    for (int j = 0; j < WJ; j++) acc[l][j] = ACC{};
+    for (int i = 0; i < WI; i++)
 #pragma unroll
-  for (int j = 0; j < WJ; j++)
+      for (int j = 0; j < WJ; j++) acc[l][i][j] = ACC{};
 #pragma unroll
-    for (int v = 0; v < 16; v++) facc[j][v] = 0.0;
+  for (int i = 0; i < WI; i++)
+#pragma unroll
+    for (int j = 0; j < WJ; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) facc[i][j][v] = 0.0;
 
   const char *Xg = reinterpret_cast<const char *>(a.X);
   const char *Yg = reinterpret_cast<const char *>(a.Y);
@@ -615,29 +651,36 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
     __syncthreads();
     if (s + NBUF - 1 < nst) issue(s + NBUF - 1);
     const char *cur = smem + (s % NBUF) * STAGE;
-    digit_stage<DT, NL, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
+    digit_stage<DT, NL, WI, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
     if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
 #pragma unroll
-      for (int j = 0; j < WJ; j++)
+      for (int i = 0; i < WI; i++)
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-          double t = 0.0;
+        for (int j = 0; j < WJ; j++)
 #pragma unroll
-          for (int l = NL - 1; l >= 0; l--) t += (double)acc[l][j][v] * rinv(l);
-          facc[j][v] += t;
-        }
+          for (int v = 0; v < 16; v++) {
+            double t = 0.0;
+#pragma unroll
+            for (int l = NL - 1; l >= 0; l--) t += (double)acc[l][i][j][v] * rinv(l);
+            facc[i][j][v] += t;
+          }
 #pragma unroll
       for (int l = 0; l < NL; l++)
 #pragma unroll
-        for (int j = 0; j < WJ; j++) acc[l][j] = ACC{};
+        for (int i = 0; i < WI; i++)
+#pragma unroll
+          for (int j = 0; j < WJ; j++) acc[l][i][j] = ACC{};
     }
   }
 
   // epilogue: combine levels in FP64
 #pragma unroll
+  for (int i = 0; i < WI; i++)
+#pragma unroll
   for (int j = 0; j < WJ; j++) {
     const int ln = yrow + j * 32 + (lane & 31);
     const int nn = n0 + ln;
+    const int xr = xrow + i * 32;
     // MODE 1: psi~[m][nn] for this lane's 16 rows, rebuilt from all S digits
     double pd[16];
     int ez = 0;
@@ -651,7 +694,7 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
         for (int g = 0; g < 4; g++) {
           double q4[4];
           const size_t off = (size_t)p * a.dbs +
-                             kb_off<DT>(min(nn, a.N - 1), m0 + xrow + 8 * g + 4 * (lane >> 5),
+                             kb_off<DT>(min(nn, a.N - 1), m0 + xr + 8 * g + 4 * (lane >> 5),
                                         (size_t)a.Kp);
           load_digit_quad<DT>(a.DB, off, q4);
 #pragma unroll
@@ -664,11 +707,11 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       double val = 0.0;
       if (DT == DT_I8) {
 #pragma unroll
-        for (int l = NL - 1; l >= 0; l--) val += (double)acc[l][j][v] * rinv(l);
+        for (int l = NL - 1; l >= 0; l--) val += (double)acc[l][i][j][v] * rinv(l);
       } else {
-        val = facc[j][v];
+        val = facc[i][j][v];
       }
-      const int lm = xrow + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+      const int lm = xr + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
       if (MODE == 0) {
         const int e = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
         a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
@@ -689,7 +732,9 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
 }
 
 template <int DT, int S, int MODE, int NLB, int TILE>
-__global__ __launch_bounds__(64 * TileCfg<TILE>::NW) void digit_nt_kernel(
+__global__ __attribute__((amdgpu_flat_work_group_size(1, DigitCfg<DT, S, MODE, NLB, TILE>::NTHR)))
+__attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
+                                   DigitCfg<DT, S, MODE, NLB, TILE>::WPE))) void digit_nt_kernel(
     oa_digit_args a, int total) {
   __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
   const int item = single_item(total);
@@ -697,7 +742,9 @@ __global__ __launch_bounds__(64 * TileCfg<TILE>::NW) void digit_nt_kernel(
 }
 
 template <int DT, int S, int MODE, int NLB, int TILE>
-__global__ __launch_bounds__(64 * TileCfg<TILE>::NW) void digit_nt_grouped(
+__global__ __attribute__((amdgpu_flat_work_group_size(1, DigitCfg<DT, S, MODE, NLB, TILE>::NTHR)))
+__attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
+                                   DigitCfg<DT, S, MODE, NLB, TILE>::WPE))) void digit_nt_grouped(
     const oa_digit_args *__restrict__ t, const int *__restrict__ prefix, int count) {
   __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
   for_items(prefix[count], [&](int item) {
