@@ -293,16 +293,23 @@ __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
 
 // Item = 64 rows x 64 columns (item % (Kp/64) selects the columns).  Thread t
 // owns columns 2(t%32), 2(t%32)+1 of rows t/32 + 8q (q < 8): coalesced
-// 16-byte reads.  DS[p] (rows np, contraction Kp) gets the S-layout digits
-// (kx0 columns zeroed, their term is exact FP64); DB[p] (rows Kp, contraction
-// np) gets all columns through an LDS transpose.  want_ds == 0 skips DS.
+// 16-byte reads.  The digits go through LDS for both layouts, so every
+// global store is 16 bytes into a contiguous run:
+//   DS[p] (rows np, contraction Kp; kx0 columns zeroed, their term is exact
+//   FP64): the item's 64 columns are 2*ES k-blocks, each 64 rows x 32 bytes
+//   = one contiguous 2 KB run;
+//   DB[p] (rows Kp, contraction np): the transposed tile, 64 k-rows of 64
+//   contraction entries.
+// want_ds == 0 skips DS.
 template <int DT, int S>
 __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
   typedef typename DTraits<DT>::elem E;
   constexpr int ES = DTraits<DT>::ES;
   constexpr int PITCH = SLT + 16;  // LDS row pitch in bytes (16-byte aligned rows)
-  // digits are integers |a| <= 127 in every type: the transpose holds int8
-  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * PITCH];
+  constexpr int B = 32 / ES;       // elements of a k-block row
+  // digits are integers |a| <= 127 in every type: LDS holds int8
+  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * PITCH];   // [col][row] for DB
+  __shared__ __attribute__((aligned(16))) int8_t ls[S][SLT * PITCH];   // [row][col] for DS
   E *DS = reinterpret_cast<E *>(a.DS);
   E *DB = reinterpret_cast<E *>(a.DB);
   const int nkt = a.Kp / SLT;
@@ -347,23 +354,35 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
     if (a.want_ds) {
 #pragma unroll
       for (int p = 0; p < S; p++) {
-        E *dst = DS + p * sds + kb_off<DT>(i, kc, a.np);
-        if constexpr (ES == 1) {
-          const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
-                                                    (((unsigned)(int)dS[1][p] & 0xffu) << 8));
-          *reinterpret_cast<unsigned short *>(dst) = u;
-        } else if constexpr (ES == 2) {
-          const E lo = to_digit<DT>(dS[0][p]), hi = to_digit<DT>(dS[1][p]);
-          *reinterpret_cast<unsigned *>(dst) =
-              (unsigned)__builtin_bit_cast(unsigned short, lo) |
-              ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
-        } else {
-          *reinterpret_cast<float2 *>(dst) = float2{(float)dS[0][p], (float)dS[1][p]};
-        }
+        const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
+                                                  (((unsigned)(int)dS[1][p] & 0xffu) << 8));
+        *reinterpret_cast<unsigned short *>(&ls[p][ii * PITCH + 2 * cp]) = u;
       }
     }
   }
   __syncthreads();
+  // DS: 16-byte chunk c (of 2*ES*128) = k-block region c/128, row (c%128)/2,
+  // half c%2; each region is one contiguous 2 KB run
+  if (a.want_ds) {
+#pragma unroll
+    for (int p = 0; p < S; p++) {
+#pragma unroll
+      for (int j = 0; j < ES; j++) {
+        const int c = t + 256 * j;
+        const int region = c / 128, row = (c % 128) / 2, half = c % 2;
+        const int col = region * B + half * (B / 2);
+        E *dst = DS + p * sds + ((size_t)(k0 / B + region) * a.np + i0 + row) * B + half * (B / 2);
+        if constexpr (ES == 1) {
+          *reinterpret_cast<i4v *>(dst) = *reinterpret_cast<const i4v *>(&ls[p][row * PITCH + col]);
+        } else {
+          E v[B / 2];
+#pragma unroll
+          for (int e = 0; e < B / 2; e++) v[e] = to_digit<DT>((double)ls[p][row * PITCH + col + e]);
+          *reinterpret_cast<i4v *>(dst) = *reinterpret_cast<const i4v *>(v);
+        }
+      }
+    }
+  }
   // DB: thread t writes 16 consecutive contraction entries (rows i) of row k0 + t/4
   const int kk = t / 4, ib = (t % 4) * 16;
 #pragma unroll
