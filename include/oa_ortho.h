@@ -151,7 +151,8 @@ int oa_overlap_sym(oa_handle_t h, int n, int npw, const double *psi,
 /* Asymmetric PairCalculator forward (the Lambda instance, alpha = 1 in the
  * reference, cpaimd.C:2510-2516): L[x][y] = alpha * sum_k psi_x[k] F_y[k]
  * over all 2*npw doubles.  Replaces ckPairCalculator.C:889 for the asymmetric
- * instance. */
+ * instance.  Slice modes split psi and F into digits with their own chunk
+ * exponents and form all tiles of the product. */
 int oa_lambda(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
               const double *F, long ldF, double alpha, int prec, double *L);
 

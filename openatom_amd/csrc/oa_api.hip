@@ -379,9 +379,16 @@ int oa_lambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, cons
   if (ldpsi < 2L * npw) return 5;
   if (ldF < 2L * npw) return 7;
   if (n == 0) return OA_SUCCESS;
-  if ((prec & 0xff) != OA_PREC_FP64) return OA_ERR_UNSUPPORTED;
   DeviceGuard g(c->device);
   const int K = 2 * npw;
+  if ((prec & 0xff) != OA_PREC_FP64) {
+    const size_t need = oa_split_lambda_ws_bytes(prec, n, K);
+    if (need == 0) return OA_ERR_UNSUPPORTED;
+    const int st = oa_ws_require(c, need);
+    if (st) return st;
+    return oa_launch_lambda_split(c->stream, prec, n, K, psi, ldpsi, F, ldF, alpha, L,
+                                  c->ws.ptr, c->ws.bytes);
+  }
   int st = oa_ws_require(c, oa_ntsplit_f64_ws_bytes(n, n, K));
   if (st) return st;
   return oa_launch_ntsplit_f64(c->stream, n, n, K, psi, ldpsi, F, ldF, alpha, L, n, c->ws.ptr,

@@ -101,6 +101,12 @@ int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K,
                             const double *psi, long ld, int kx0_end,
                             double alpha, double *S, double diag,
                             double *maxdev, void *ws, size_t ws_bytes);
+// Asymmetric split product L[x][y] = alpha * sum_k psi[x][k] F[y][k] (the
+// Lambda forward) on digit slices of psi and F, all tiles, split-K.
+size_t oa_split_lambda_ws_bytes(int prec, int n, int K);
+int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *psi, long ld,
+                           const double *F, long ldF, double alpha, double *L, void *ws,
+                           size_t ws_bytes);
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K,
                             const double *M, int transM, const double *psi,
                             long ldpsi, int kx0_end, double alpha, double beta,
@@ -134,6 +140,7 @@ struct oa_slice_args {
   const unsigned long long *maxS, *maxB;
   void *DS, *DB;
   int np, Kp, want_ds, vec;
+  int want_db, pad;
 };
 struct oa_digit_args {
   const void *X;  // k-blocked digits, rows m

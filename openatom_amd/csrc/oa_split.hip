@@ -40,7 +40,9 @@
 //   slice_rows    an n x n matrix -> digits with one exponent per row.
 //   digit_nt      C[m][n] = sum_k X[m][k] Y[n][k] over digit slices:
 //                 MODE 0 = split-K partial tiles of S (upper triangle),
-//                 MODE 1 = back-multiply epilogue out = a*T*psi + b*out.
+//                 MODE 1 = back-multiply epilogue out = a*T*psi + b*out,
+//                 MODE 2 = split-K partial tiles of psi F^T (all tiles, the
+//                 asymmetric Lambda product).
 //                 TILE 128: 4 waves as 2 x 2 with 64 x 64 wave tiles (int8,
 //                 up to 5 levels; one wave per SIMD, up to 512 registers), else
 //                 8 waves as 4 x 2 with 32 x 64 wave tiles;
@@ -300,7 +302,7 @@ __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
 //   = one contiguous 2 KB run;
 //   DB[p] (rows Kp, contraction np): the transposed tile, 64 k-rows of 64
 //   contraction entries.
-// want_ds == 0 skips DS.
+// want_ds == 0 skips DS, want_db == 0 skips DB.
 template <int DT, int S>
 __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
   typedef typename DTraits<DT>::elem E;
@@ -387,6 +389,7 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
   const int kk = t / 4, ib = (t % 4) * 16;
 #pragma unroll
   for (int p = 0; p < S; p++) {
+    if (!a.want_db) break;
     E *dst = DB + p * sdb;
     const i4v src = *reinterpret_cast<const i4v *>(&lt[p][kk * PITCH + ib]);
     if constexpr (ES == 1) {
@@ -556,7 +559,7 @@ __device__ __forceinline__ void load_digit_quad(const void *base, size_t off, do
 
 template <int DT, int S, int MODE, int NLB, int TILE>
 struct DigitCfg {
-  static constexpr int NL = (MODE == 0) ? S : NLB;  // slices in the product and levels kept
+  static constexpr int NL = (MODE != 1) ? S : NLB;  // slices in the product and levels kept
   // int32 accumulators of up to 5 levels fit a 64 x 64 wave tile; FP32 ones
   // carry an FP64 promotion copy and stay on 32 x 64
   static constexpr bool WIDE = (TILE == 128 && DT == DT_I8 && NL <= 5);
@@ -610,14 +613,20 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
     tile = logical % a.ntiles;
     z = logical / a.ntiles;
     upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
+  } else if (MODE == 2) {
+    const int T = cdv(a.M, TILE);
+    tile = logical % a.ntiles;
+    z = logical / a.ntiles;
+    bi = tile % T;
+    bj = tile / T;
   } else {
     const int tm = cdv(a.M, TILE);
     bi = logical % tm;
     bj = logical / tm;
   }
   const int m0 = bi * TILE, n0 = bj * TILE;
-  const int ka = (MODE == 0) ? z * a.kc : 0;
-  const int kb = (MODE == 0) ? min(a.kfull, ka + a.kc) : a.kfull;
+  const int ka = (MODE != 1) ? z * a.kc : 0;
+  const int kb = (MODE != 1) ? min(a.kfull, ka + a.kc) : a.kfull;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xrow = (w / WC) * 32 * WI, yrow = (w % WC) * 32 * WJ;
 
@@ -735,6 +744,11 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
         const int e = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
         a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
             ldexp(val, 2 * (e - 7));
+      } else if (MODE == 2) {
+        const int ex = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
+        const int ey = chunk_exp(__longlong_as_double((long long)a.colmax[z]));
+        a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
+            ldexp(val, ex + ey - 14);
       } else {
         const int m = m0 + lm;
         if (m < a.M && nn < a.N) {
@@ -818,6 +832,24 @@ __global__ __launch_bounds__(256) void split_reduce_grouped(const oa_sreduce_arg
     const int i = group_find(prefix, count, item);
     split_reduce_body<TILE>(t[i], item - prefix[i]);
   });
+}
+
+// Asymmetric reduce: L[i][j] = alpha * sum_z part (fixed order), all tiles.
+template <int TILE>
+__global__ __launch_bounds__(256) void lambda_reduce_kernel(const double *__restrict__ part,
+                                                            int n, int nz, double alpha,
+                                                            double *L) {
+  const int T = cdv(n, TILE);
+  const int ntiles = T * T;
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const int tile = (int)(idx / (TILE * TILE));
+  if (tile >= ntiles) return;
+  const int e = (int)(idx % (TILE * TILE));
+  const int i = (tile % T) * TILE + e / TILE, j = (tile / T) * TILE + e % TILE;
+  if (i >= n || j >= n) return;
+  double s = 0.0;
+  for (int z = 0; z < nz; z++) s += part[((long)z * ntiles + tile) * (TILE * TILE) + e];
+  L[(long)i * n + j] = alpha * s;
 }
 
 // ---------------------------------------------------------------- host plan
@@ -958,7 +990,7 @@ void fill_inst(const oa_split_plan &pl, int n, int K, const double *psi, long ld
   r->cm = oa_chunkmax_args{psi, ld, n, K, kx0_end, pl.kchunk, pl.nz, 0, maxS, maxB};
   r->items_cm = pl.nz * cdv(n, 8);
   r->sl = oa_slice_args{psi, ld, n, K, kx0_end, pl.kchunk, maxS, maxB, ws + pl.off_ds,
-                        ws + pl.off_db, pl.np, pl.Kp, 1, vec};
+                        ws + pl.off_db, pl.np, pl.Kp, 1, vec, 1, 0};
   r->items_sl = (pl.Kp / SLT) * (pl.np / SLT);
   oa_digit_args &a = r->d0;
   a = oa_digit_args{};
@@ -1052,6 +1084,89 @@ int run_backmul(hipStream_t s, const oa_split_plan &pl, int n, int K, const doub
   OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 1, NLB, 128>), (DigitCfg<DT, S, 1, NLB, 128>::NTHR),
                   r.items_d1, s, r.d1);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+// Workspace of the asymmetric (Lambda) product: digits of psi and of F in
+// the S layout (all columns), chunk maxima of both, partial tiles.
+struct LambdaPlan {
+  oa_split_plan pl;
+  int ntiles;
+  size_t off_x, off_y, off_mx, off_my, off_part, total;
+};
+
+LambdaPlan make_lambda_plan(int prec, int n, int K) {
+  LambdaPlan lp;
+  lp.pl = make_plan(prec, n, K, 128);
+  const oa_split_plan &pl = lp.pl;
+  const int T = pl.np / 128;
+  lp.ntiles = T * T;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  lp.off_x = take((size_t)pl.S * pl.np * pl.Kp * pl.es);
+  lp.off_y = take((size_t)pl.S * pl.np * pl.Kp * pl.es);
+  lp.off_mx = take((size_t)2 * pl.nz * 8);
+  lp.off_my = take((size_t)2 * pl.nz * 8);
+  lp.off_part = take((size_t)pl.nz * lp.ntiles * 128 * 128 * 8);
+  lp.total = off;
+  return lp;
+}
+
+template <int DT, int S>
+int run_lambda(hipStream_t s, const LambdaPlan &lp, int n, int K, const double *psi, long ld,
+               const double *F, long ldF, double alpha, double *L, char *ws) {
+  const oa_split_plan &pl = lp.pl;
+  auto *mx = reinterpret_cast<unsigned long long *>(ws + lp.off_mx);
+  auto *my = reinterpret_cast<unsigned long long *>(ws + lp.off_my);
+  hipLaunchKernelGGL(fill_u64, dim3(cdv(4 * pl.nz, 256)), dim3(256), 0, s, mx, 2L * pl.nz);
+  hipLaunchKernelGGL(fill_u64, dim3(cdv(4 * pl.nz, 256)), dim3(256), 0, s, my, 2L * pl.nz);
+  const int items_cm = pl.nz * cdv(n, 8), items_sl = (pl.Kp / SLT) * (pl.np / SLT);
+  const struct { const double *x; long ld; unsigned long long *m; size_t off; } ops[2] = {
+      {psi, ld, mx, lp.off_x}, {F, ldF, my, lp.off_y}};
+  for (const auto &o : ops) {
+    const int vec = (((uintptr_t)o.x & 15) == 0 && (o.ld % 2) == 0) ? 1 : 0;
+    oa_chunkmax_args cm{o.x, o.ld, n, K, 0, pl.kchunk, pl.nz, 0, o.m, o.m + pl.nz};
+    OA_LAUNCH_ITEMS(chunk_max_kernel, 256, items_cm, s, cm);
+    oa_slice_args sl{o.x, o.ld, n, K, 0, pl.kchunk, o.m, o.m + pl.nz, ws + o.off, nullptr,
+                     pl.np, pl.Kp, 1, vec, 0, 0};
+    OA_LAUNCH_ITEMS((slice_psi_kernel<DT, S>), 256, items_sl, s, sl);
+  }
+  oa_digit_args a{};
+  a.X = ws + lp.off_x;
+  a.Y = ws + lp.off_y;
+  a.ldx = a.ldy = pl.np;
+  a.sx = a.sy = (long)pl.np * pl.Kp;
+  a.M = n;
+  a.N = n;
+  a.kc = pl.kchunk;
+  a.kfull = pl.Kp;
+  a.ntiles = lp.ntiles;
+  a.nz = pl.nz;
+  a.part = reinterpret_cast<double *>(ws + lp.off_part);
+  a.chunk_max = mx;
+  a.colmax = my;
+  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 2, S - 1, 128>),
+                  (DigitCfg<DT, S, 2, S - 1, 128>::NTHR), lp.ntiles * pl.nz, s, a);
+  const long total = (long)lp.ntiles * 128 * 128;
+  hipLaunchKernelGGL(lambda_reduce_kernel<128>, dim3((unsigned)cdv((int)total, 256)), dim3(256),
+                     0, s, (const double *)a.part, n, pl.nz, alpha, L);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+template <int DT>
+int dispatch_lambda(hipStream_t st, const LambdaPlan &lp, int n, int K, const double *psi,
+                    long ld, const double *F, long ldF, double alpha, double *L, char *ws) {
+  switch (lp.pl.S) {
+    case 2: return run_lambda<DT, 2>(st, lp, n, K, psi, ld, F, ldF, alpha, L, ws);
+    case 3: return run_lambda<DT, 3>(st, lp, n, K, psi, ld, F, ldF, alpha, L, ws);
+    case 4: return run_lambda<DT, 4>(st, lp, n, K, psi, ld, F, ldF, alpha, L, ws);
+    case 5: return run_lambda<DT, 5>(st, lp, n, K, psi, ld, F, ldF, alpha, L, ws);
+    case 6: return run_lambda<DT, 6>(st, lp, n, K, psi, ld, F, ldF, alpha, L, ws);
+  }
+  return OA_ERR_UNSUPPORTED;
 }
 
 template <int DT>
@@ -1189,6 +1304,26 @@ int oa_launch_overlap_split(hipStream_t s, int prec, int n, int K, const double 
   if (dt == DT_F16) return dispatch_overlap<DT_F16>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
   if (dt == DT_F32) return dispatch_overlap<DT_F32>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
   return dispatch_overlap<DT_BF16>(s, pl, n, K, psi, ld, kx0_end, alpha, S, diag, maxdev, w);
+}
+
+size_t oa_split_lambda_ws_bytes(int prec, int n, int K) {
+  int dt, S;
+  if ((prec & 0xff) == OA_PREC_FP64 || !decode_prec(prec & 0xffff, dt, S)) return 0;
+  return make_lambda_plan(prec & 0xffff, n, K).total;
+}
+
+int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *psi, long ld,
+                           const double *F, long ldF, double alpha, double *L, void *ws,
+                           size_t ws_bytes) {
+  int dt, Sl;
+  if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
+  const LambdaPlan lp = make_lambda_plan(prec & 0xffff, n, K);
+  if (lp.total > ws_bytes) return OA_ERR_CAPACITY;
+  char *w = (char *)ws;
+  if (dt == DT_I8) return dispatch_lambda<DT_I8>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
+  if (dt == DT_F16) return dispatch_lambda<DT_F16>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
+  if (dt == DT_F32) return dispatch_lambda<DT_F32>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
+  return dispatch_lambda<DT_BF16>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
 }
 
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double *M, int transM,

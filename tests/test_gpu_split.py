@@ -125,3 +125,20 @@ def test_split_step_meets_residual_target(handle, prec):
     print(f"residual {prec} {res:.3e}  fp64 {res64:.3e}  |dT| {(Ts - T64).abs().max().item():.3e}")
     assert res <= 1e-10
     assert (Ts - T64).abs().max().item() <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["int8x5", "fp16x5", "int8x3"])
+def test_split_lambda(handle, prec):
+    """Asymmetric Lambda = psi F^T on digits of psi and F (own chunk
+    exponents), all tiles, against the FP64 product; F has a different scale
+    from psi (std 0.1) and no orthogonality."""
+    from openatom_amd.workload import make_force
+    psi = make_states(200, 6000, eps=1e-3, seed=5, device=DEV, rng="torch")
+    F = make_force(200, 6000, seed=6, device=DEV, rng="torch")
+    L64 = handle.lambda_fwd(psi, F, alpha=1.0, prec="fp64")
+    Ls = handle.lambda_fwd(psi, F, alpha=1.0, prec=prec)
+    torch.cuda.synchronize()
+    e = rel(Ls, L64)
+    print(f"{prec}: Lambda max rel error {e:.3e}")
+    slices = int(prec.split("x")[1])
+    assert e < 10 * TOL[slices]
