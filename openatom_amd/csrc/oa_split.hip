@@ -43,9 +43,8 @@
 //                 MODE 1 = back-multiply epilogue out = a*T*psi + b*out,
 //                 MODE 2 = split-K partial tiles of psi F^T (all tiles, the
 //                 asymmetric Lambda product).
-//                 TILE 128: 4 waves as 2 x 2 with 64 x 64 wave tiles (int8,
-//                 up to 5 levels; one wave per SIMD, up to 512 registers), else
-//                 8 waves as 4 x 2 with 32 x 64 wave tiles;
+//                 TILE 128: 8 waves as 4 x 2 with 32 x 64 wave tiles, two
+//                 waves per SIMD;
 //                 TILE 64: 256 threads, 4 waves as 2 x 2, wave tile 32 x 32.
 //   split_reduce  fixed-order sum of the partial tiles, FP64 G=0 term,
 //                 mirror, max|S - d I|.
@@ -65,6 +64,10 @@ constexpr int DBK = 32;   // chunk granularity (elements); a stage holds 32 byte
 constexpr int SMAX = 6;
 constexpr int SLT = 64;   // slicing tile (rows x columns)
 constexpr long KCHUNK_TARGET = 8192;  // columns per scale block / split-K chunk
+// Back-multiply tiles per workgroup run.  Runs of 4 row tiles (one column
+// tile, shared psi panel) measured slower than one tile per workgroup: eight
+// concurrent workgroups on one L2 then share each psi panel.
+constexpr int BM_RUN = 1;
 
 __host__ __device__ inline int cdv(int a, int b) { return (a + b - 1) / b; }
 
@@ -245,12 +248,36 @@ __device__ __forceinline__ void chunk_max_body(const oa_chunkmax_args &a, int it
   const int z = item % a.nz, r0 = (item / a.nz) * 8;
   const int k0 = z * a.kchunk, k1 = min(a.K, k0 + a.kchunk);
   double ms = 0.0, mb = 0.0;
+  const bool vec = ((a.ld & 1) == 0) && (((uintptr_t)a.psi & 15) == 0) && ((k0 & 1) == 0);
   for (int r = r0; r < min(a.n, r0 + 8); r++) {
     const double *row = a.psi + (long)r * a.ld;
-    for (int k = k0 + threadIdx.x; k < k1; k += 256) {
-      const double v = fabs(row[k]);
-      mb = fmax(mb, v);
-      if (k >= a.kx0_end) ms = fmax(ms, v);
+    int k = k0 + 2 * (int)threadIdx.x;
+    if (vec) {
+      // 16-byte loads, four in flight per thread
+      for (; k + 3 * 512 + 1 < k1; k += 4 * 512) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = *reinterpret_cast<const double2 *>(row + k + u * 512);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const double x0 = fabs(v[u].x), x1 = fabs(v[u].y);
+          const int kk = k + u * 512;
+          mb = fmax(mb, fmax(x0, x1));
+          if (kk >= a.kx0_end) ms = fmax(ms, x0);
+          if (kk + 1 >= a.kx0_end) ms = fmax(ms, x1);
+        }
+      }
+    }
+    // remainder (and the unaligned case): this thread's pairs (k, k+1)
+    for (; k < k1; k += 512) {
+      const double x0 = fabs(row[k]);
+      mb = fmax(mb, x0);
+      if (k >= a.kx0_end) ms = fmax(ms, x0);
+      if (k + 1 < k1) {
+        const double x1 = fabs(row[k + 1]);
+        mb = fmax(mb, x1);
+        if (k + 1 >= a.kx0_end) ms = fmax(ms, x1);
+      }
     }
   }
   __shared__ double rs[256], rb[256];
@@ -295,23 +322,18 @@ __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
 
 // Item = 64 rows x 64 columns (item % (Kp/64) selects the columns).  Thread t
 // owns columns 2(t%32), 2(t%32)+1 of rows t/32 + 8q (q < 8): coalesced
-// 16-byte reads.  The digits go through LDS for both layouts, so every
-// global store is 16 bytes into a contiguous run:
-//   DS[p] (rows np, contraction Kp; kx0 columns zeroed, their term is exact
-//   FP64): the item's 64 columns are 2*ES k-blocks, each 64 rows x 32 bytes
-//   = one contiguous 2 KB run;
-//   DB[p] (rows Kp, contraction np): the transposed tile, 64 k-rows of 64
-//   contraction entries.
-// want_ds == 0 skips DS, want_db == 0 skips DB.
+// 16-byte reads.  DS[p] (rows np, contraction Kp) gets the S-layout digits
+// (kx0 columns zeroed, their term is exact FP64); DB[p] (rows Kp, contraction
+// np) gets all columns through an LDS transpose.  want_ds == 0 skips DS,
+// want_db == 0 skips DB.  (Staging DS through LDS as well, for 16-byte stores,
+// measured slower: the second buffer halves the resident workgroups.)
 template <int DT, int S>
 __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
   typedef typename DTraits<DT>::elem E;
   constexpr int ES = DTraits<DT>::ES;
   constexpr int PITCH = SLT + 16;  // LDS row pitch in bytes (16-byte aligned rows)
-  constexpr int B = 32 / ES;       // elements of a k-block row
-  // digits are integers |a| <= 127 in every type: LDS holds int8
-  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * PITCH];   // [col][row] for DB
-  __shared__ __attribute__((aligned(16))) int8_t ls[S][SLT * PITCH];   // [row][col] for DS
+  // digits are integers |a| <= 127 in every type: the transpose holds int8
+  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * PITCH];
   E *DS = reinterpret_cast<E *>(a.DS);
   E *DB = reinterpret_cast<E *>(a.DB);
   const int nkt = a.Kp / SLT;
@@ -322,19 +344,25 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
   const int z = min(kc, a.K - 1 > 0 ? a.K - 1 : 0) / a.kchunk;
   const double sS = chunk_scale(a.maxS, z), sB = chunk_scale(a.maxB, z);
   const size_t sds = (size_t)a.np * a.Kp, sdb = (size_t)a.Kp * a.np;
-#pragma unroll 2
+  // all eight 16-byte loads first (eight in flight per thread)
+  double xq[8][2];
+#pragma unroll
   for (int q = 0; q < 8; q++) {
-    const int ii = rg + 8 * q, i = i0 + ii;
-    double x[2];
+    const int i = i0 + rg + 8 * q;
     if (a.vec && i < a.n && kc + 1 < a.K) {
       const double2 v = *reinterpret_cast<const double2 *>(a.psi + (long)i * a.ld + kc);
-      x[0] = v.x;
-      x[1] = v.y;
+      xq[q][0] = v.x;
+      xq[q][1] = v.y;
     } else {
 #pragma unroll
       for (int e = 0; e < 2; e++)
-        x[e] = (i < a.n && kc + e < a.K) ? a.psi[(long)i * a.ld + kc + e] : 0.0;
+        xq[q][e] = (i < a.n && kc + e < a.K) ? a.psi[(long)i * a.ld + kc + e] : 0.0;
     }
+  }
+#pragma unroll 2
+  for (int q = 0; q < 8; q++) {
+    const int ii = rg + 8 * q, i = i0 + ii;
+    const double x[2] = {xq[q][0], xq[q][1]};
     double dB[2][S], dS[2][S];
 #pragma unroll
     for (int e = 0; e < 2; e++) {
@@ -356,35 +384,23 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
     if (a.want_ds) {
 #pragma unroll
       for (int p = 0; p < S; p++) {
-        const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
-                                                  (((unsigned)(int)dS[1][p] & 0xffu) << 8));
-        *reinterpret_cast<unsigned short *>(&ls[p][ii * PITCH + 2 * cp]) = u;
-      }
-    }
-  }
-  __syncthreads();
-  // DS: 16-byte chunk c (of 2*ES*128) = k-block region c/128, row (c%128)/2,
-  // half c%2; each region is one contiguous 2 KB run
-  if (a.want_ds) {
-#pragma unroll
-    for (int p = 0; p < S; p++) {
-#pragma unroll
-      for (int j = 0; j < ES; j++) {
-        const int c = t + 256 * j;
-        const int region = c / 128, row = (c % 128) / 2, half = c % 2;
-        const int col = region * B + half * (B / 2);
-        E *dst = DS + p * sds + ((size_t)(k0 / B + region) * a.np + i0 + row) * B + half * (B / 2);
+        E *dst = DS + p * sds + kb_off<DT>(i, kc, a.np);
         if constexpr (ES == 1) {
-          *reinterpret_cast<i4v *>(dst) = *reinterpret_cast<const i4v *>(&ls[p][row * PITCH + col]);
+          const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
+                                                    (((unsigned)(int)dS[1][p] & 0xffu) << 8));
+          *reinterpret_cast<unsigned short *>(dst) = u;
+        } else if constexpr (ES == 2) {
+          const E lo = to_digit<DT>(dS[0][p]), hi = to_digit<DT>(dS[1][p]);
+          *reinterpret_cast<unsigned *>(dst) =
+              (unsigned)__builtin_bit_cast(unsigned short, lo) |
+              ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
         } else {
-          E v[B / 2];
-#pragma unroll
-          for (int e = 0; e < B / 2; e++) v[e] = to_digit<DT>((double)ls[p][row * PITCH + col + e]);
-          *reinterpret_cast<i4v *>(dst) = *reinterpret_cast<const i4v *>(v);
+          *reinterpret_cast<float2 *>(dst) = float2{(float)dS[0][p], (float)dS[1][p]};
         }
       }
     }
   }
+  __syncthreads();
   // DB: thread t writes 16 consecutive contraction entries (rows i) of row k0 + t/4
   const int kk = t / 4, ib = (t % 4) * 16;
 #pragma unroll
@@ -560,16 +576,18 @@ __device__ __forceinline__ void load_digit_quad(const void *base, size_t off, do
 template <int DT, int S, int MODE, int NLB, int TILE>
 struct DigitCfg {
   static constexpr int NL = (MODE != 1) ? S : NLB;  // slices in the product and levels kept
-  // int32 accumulators of up to 5 levels fit a 64 x 64 wave tile; FP32 ones
-  // carry an FP64 promotion copy and stay on 32 x 64
-  static constexpr bool WIDE = (TILE == 128 && DT == DT_I8 && NL <= 5);
+  // 64 x 64 wave tiles (one wave per SIMD) fit int32 accumulators of up to
+  // 5 levels but measured slower than two 32 x 64 waves per SIMD (latency is
+  // hidden by the second wave, not by the wider tile): kept off
+  static constexpr bool WIDE = false;
   typedef TileCfg<TILE, WIDE> T;
   static constexpr int WI = T::WI, WJ = T::WJ, WPE = T::WPE;
   static constexpr int PANEL = TILE * 32;           // bytes of one slice panel per stage
   static constexpr int STAGE = 2 * NL * PANEL;      // X and Y slices of one stage
-  // LDS ring depth: four stages (three in flight) where they fit the 160 KB
-  // of a workgroup, else three
-  static constexpr int NBUF = (4 * STAGE <= 160 * 1024) ? 4 : 3;
+  // LDS ring depth: as many stages (up to five, NBUF - 1 in flight) as fit
+  // the 160 KB of a workgroup
+  static constexpr int NBUF =
+      (5 * STAGE <= 160 * 1024) ? 5 : ((4 * STAGE <= 160 * 1024) ? 4 : 3);
   static constexpr int SMEM = NBUF * STAGE;
   static constexpr int NW = T::NW;
   static constexpr int NTHR = 64 * NW;
@@ -584,8 +602,13 @@ struct DigitCfg {
 //           digits of DB.
 // The X and Y panels of NBUF stages rotate through LDS, filled by
 // direct-to-LDS loads (global_load_lds_dwordx4) NBUF-1 stages ahead.
-template <int DT, int S, int MODE, int NLB, int TILE>
-__device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, char *smem) {
+// MODE 1 tiles all span the whole contraction, so a workgroup may take a run
+// of TPW consecutive tiles (row tiles of one column tile: the Y panel is
+// shared) as one continuous stage sequence: the ring never drains between
+// tiles and the next tile's first loads are in flight during the epilogue.
+template <int DT, int S, int MODE, int NLB, int TILE, int TPW = 1>
+__device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, int total,
+                                           char *smem) {
   typedef typename DTraits<DT>::acc_t ACC;
   typedef DigitCfg<DT, S, MODE, NLB, TILE> C;
   constexpr int NL = C::NL;
@@ -608,23 +631,30 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   // FP64 sum per output
   constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 128 / SK;
 
-  int bi, bj, z = 0, tile = logical;
-  if (MODE == 0) {
-    tile = logical % a.ntiles;
-    z = logical / a.ntiles;
-    upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
-  } else if (MODE == 2) {
-    const int T = cdv(a.M, TILE);
-    tile = logical % a.ntiles;
-    z = logical / a.ntiles;
-    bi = tile % T;
-    bj = tile / T;
-  } else {
-    const int tm = cdv(a.M, TILE);
-    bi = logical % tm;
-    bj = logical / tm;
-  }
-  const int m0 = bi * TILE, n0 = bj * TILE;
+  static_assert(TPW == 1 || MODE == 1, "tile runs need a common contraction range");
+  // output tile `logical`: its (row, column) tile and split index
+  auto coords = [&](int logical, int &bi, int &bj, int &z, int &tile) {
+    z = 0;
+    tile = logical;
+    if (MODE == 0) {
+      tile = logical % a.ntiles;
+      z = logical / a.ntiles;
+      upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
+    } else if (MODE == 2) {
+      const int T = cdv(a.M, TILE);
+      tile = logical % a.ntiles;
+      z = logical / a.ntiles;
+      bi = tile % T;
+      bj = tile / T;
+    } else {
+      const int tm = cdv(a.M, TILE);
+      bi = logical % tm;
+      bj = logical / tm;
+    }
+  };
+  const int ntr = min(TPW, total - first);  // tiles in this run
+  int bi, bj, z, tile;
+  coords(first, bi, bj, z, tile);
   const int ka = (MODE != 1) ? z * a.kc : 0;
   const int kb = (MODE != 1) ? min(a.kfull, ka + a.kc) : a.kfull;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -651,9 +681,14 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   // Wave-load i of a stage: operand o, slice p, quarter q (rows 32q..32q+31).
   // Lane L fills LDS slot q*64 + L of the panel, i.e. row r = (q*64+L)/2,
   // stored half hs = L%2, which holds global half hs ^ ((r >> 3) & 1).
-  auto issue = [&](int st) {
+  const int nst = (kb - ka) / SK;  // stages per tile
+  auto issue = [&](int gs) {
+    const int t = (TPW == 1) ? 0 : gs / nst, st = (TPW == 1) ? gs : gs - t * nst;
+    int ti = bi, tj = bj, tz, tt;
+    if (TPW > 1) coords(first + t, ti, tj, tz, tt);
+    const int m0 = ti * TILE, n0 = tj * TILE;
     const int kblk = (ka + st * SK) / SK;
-    char *buf = smem + (st % NBUF) * STAGE;
+    char *buf = smem + (gs % NBUF) * STAGE;
 #pragma unroll
     for (int j = 0; j < PERW; j++) {
       const int i = w + NW * j;
@@ -666,19 +701,21 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
     }
   };
-  const int nst = (kb - ka) / SK;
+  const int ngs = nst * ntr;  // stages of the run
 #pragma unroll
   for (int st = 0; st < NBUF - 1; st++)
-    if (st < nst) issue(st);
-  for (int s = 0; s < nst; s++) {
-    // stages s+1 .. min(nst-1, s+NBUF-2) may stay in flight
-    const int ahead = min(nst - 1 - s, NBUF - 2);
-    if (NBUF == 4 && ahead >= 2) wait_vm<2 * PERW>();
+    if (st < ngs) issue(st);
+  for (int gs = 0; gs < ngs; gs++) {
+    const int s = (TPW == 1) ? gs : gs % nst;  // stage within the tile
+    // stages gs+1 .. min(ngs-1, gs+NBUF-2) may stay in flight
+    const int ahead = min(ngs - 1 - gs, NBUF - 2);
+    if (NBUF >= 5 && ahead >= 3) wait_vm<3 * PERW>();
+    else if (NBUF >= 4 && ahead >= 2) wait_vm<2 * PERW>();
     else if (ahead >= 1) wait_vm<PERW>();
     else wait_vm<0>();
     __syncthreads();
-    if (s + NBUF - 1 < nst) issue(s + NBUF - 1);
-    const char *cur = smem + (s % NBUF) * STAGE;
+    if (gs + NBUF - 1 < ngs) issue(gs + NBUF - 1);
+    const char *cur = smem + (gs % NBUF) * STAGE;
     digit_stage<DT, NL, WI, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
     if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
 #pragma unroll
@@ -699,9 +736,11 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
 #pragma unroll
           for (int j = 0; j < WJ; j++) acc[l][i][j] = ACC{};
     }
-  }
+    if (s != nst - 1) continue;
+    if (TPW > 1) coords(first + gs / nst, bi, bj, z, tile);
+    const int m0 = bi * TILE, n0 = bj * TILE;
 
-  // epilogue: combine levels in FP64
+  // epilogue of the tile: combine levels in FP64
 #pragma unroll
   for (int i = 0; i < WI; i++)
 #pragma unroll
@@ -760,18 +799,36 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       }
     }
   }
+    if (TPW > 1) {
+      // the run continues: clear the accumulators for the next tile
+#pragma unroll
+      for (int l = 0; l < NL; l++)
+#pragma unroll
+        for (int i = 0; i < WI; i++)
+#pragma unroll
+          for (int j = 0; j < WJ; j++) acc[l][i][j] = ACC{};
+#pragma unroll
+      for (int i = 0; i < WI; i++)
+#pragma unroll
+        for (int j = 0; j < WJ; j++)
+#pragma unroll
+          for (int v = 0; v < 16; v++) facc[i][j][v] = 0.0;
+    }
+  }
   // the next item's first loads overwrite the ring
   __syncthreads();
 }
 
-template <int DT, int S, int MODE, int NLB, int TILE>
+template <int DT, int S, int MODE, int NLB, int TILE, int TPW = 1>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, DigitCfg<DT, S, MODE, NLB, TILE>::NTHR)))
 __attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
                                    DigitCfg<DT, S, MODE, NLB, TILE>::WPE))) void digit_nt_kernel(
     oa_digit_args a, int total) {
+  // workgroup g takes the run of tiles [g*TPW, g*TPW + TPW)
   __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
-  const int item = single_item(total);
-  if (item < total) digit_body<DT, S, MODE, NLB, TILE>(a, item, smem);
+  const int runs = cdv(total, TPW);
+  const int g = single_item(runs);
+  if (g < runs) digit_body<DT, S, MODE, NLB, TILE, TPW>(a, g * TPW, total, smem);
 }
 
 template <int DT, int S, int MODE, int NLB, int TILE>
@@ -782,7 +839,8 @@ __attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
   __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
   for_items(prefix[count], [&](int item) {
     const int i = group_find(prefix, count, item);
-    digit_body<DT, S, MODE, NLB, TILE>(t[i], item - prefix[i], smem);
+    const int local = item - prefix[i];
+    digit_body<DT, S, MODE, NLB, TILE>(t[i], local, local + 1, smem);
   });
 }
 
@@ -1081,8 +1139,10 @@ int run_backmul(hipStream_t s, const oa_split_plan &pl, int n, int K, const doub
   r.d1.alpha = alpha;
   if (!reuse) launch_slice_psi<DT, S>(s, pl, r, false);
   OA_LAUNCH_ITEMS((slice_rows_kernel<DT, NLB>), 256, r.items_rows, s, r.rows);
-  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 1, NLB, 128>), (DigitCfg<DT, S, 1, NLB, 128>::NTHR),
-                  r.items_d1, s, r.d1);
+  // runs of BM_RUN row tiles of one column tile per workgroup
+  const int runs = cdv(r.items_d1, BM_RUN);
+  hipLaunchKernelGGL((digit_nt_kernel<DT, S, 1, NLB, 128, BM_RUN>), dim3(cdv(runs, 8) * 8),
+                     dim3(DigitCfg<DT, S, 1, NLB, 128>::NTHR), 0, s, r.d1, r.items_d1);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
