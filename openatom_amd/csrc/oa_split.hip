@@ -64,10 +64,6 @@ constexpr int DBK = 32;   // chunk granularity (elements); a stage holds 32 byte
 constexpr int SMAX = 6;
 constexpr int SLT = 64;   // slicing tile (rows x columns)
 constexpr long KCHUNK_TARGET = 8192;  // columns per scale block / split-K chunk
-// Back-multiply tiles per workgroup run.  Runs of 4 row tiles (one column
-// tile, shared psi panel) measured slower than one tile per workgroup: eight
-// concurrent workgroups on one L2 then share each psi panel.
-constexpr int BM_RUN = 1;
 
 __host__ __device__ inline int cdv(int a, int b) { return (a + b - 1) / b; }
 
@@ -602,13 +598,8 @@ struct DigitCfg {
 //           digits of DB.
 // The X and Y panels of NBUF stages rotate through LDS, filled by
 // direct-to-LDS loads (global_load_lds_dwordx4) NBUF-1 stages ahead.
-// MODE 1 tiles all span the whole contraction, so a workgroup may take a run
-// of TPW consecutive tiles (row tiles of one column tile: the Y panel is
-// shared) as one continuous stage sequence: the ring never drains between
-// tiles and the next tile's first loads are in flight during the epilogue.
-template <int DT, int S, int MODE, int NLB, int TILE, int TPW = 1>
-__device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, int total,
-                                           char *smem) {
+template <int DT, int S, int MODE, int NLB, int TILE>
+__device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, char *smem) {
   typedef typename DTraits<DT>::acc_t ACC;
   typedef DigitCfg<DT, S, MODE, NLB, TILE> C;
   constexpr int NL = C::NL;
@@ -631,30 +622,23 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, in
   // FP64 sum per output
   constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 128 / SK;
 
-  static_assert(TPW == 1 || MODE == 1, "tile runs need a common contraction range");
-  // output tile `logical`: its (row, column) tile and split index
-  auto coords = [&](int logical, int &bi, int &bj, int &z, int &tile) {
-    z = 0;
-    tile = logical;
-    if (MODE == 0) {
-      tile = logical % a.ntiles;
-      z = logical / a.ntiles;
-      upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
-    } else if (MODE == 2) {
-      const int T = cdv(a.M, TILE);
-      tile = logical % a.ntiles;
-      z = logical / a.ntiles;
-      bi = tile % T;
-      bj = tile / T;
-    } else {
-      const int tm = cdv(a.M, TILE);
-      bi = logical % tm;
-      bj = logical / tm;
-    }
-  };
-  const int ntr = min(TPW, total - first);  // tiles in this run
-  int bi, bj, z, tile;
-  coords(first, bi, bj, z, tile);
+  int bi, bj, z = 0, tile = logical;
+  if (MODE == 0) {
+    tile = logical % a.ntiles;
+    z = logical / a.ntiles;
+    upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
+  } else if (MODE == 2) {
+    const int T = cdv(a.M, TILE);
+    tile = logical % a.ntiles;
+    z = logical / a.ntiles;
+    bi = tile % T;
+    bj = tile / T;
+  } else {
+    const int tm = cdv(a.M, TILE);
+    bi = logical % tm;
+    bj = logical / tm;
+  }
+  const int m0 = bi * TILE, n0 = bj * TILE;
   const int ka = (MODE != 1) ? z * a.kc : 0;
   const int kb = (MODE != 1) ? min(a.kfull, ka + a.kc) : a.kfull;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -681,14 +665,9 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, in
   // Wave-load i of a stage: operand o, slice p, quarter q (rows 32q..32q+31).
   // Lane L fills LDS slot q*64 + L of the panel, i.e. row r = (q*64+L)/2,
   // stored half hs = L%2, which holds global half hs ^ ((r >> 3) & 1).
-  const int nst = (kb - ka) / SK;  // stages per tile
-  auto issue = [&](int gs) {
-    const int t = (TPW == 1) ? 0 : gs / nst, st = (TPW == 1) ? gs : gs - t * nst;
-    int ti = bi, tj = bj, tz, tt;
-    if (TPW > 1) coords(first + t, ti, tj, tz, tt);
-    const int m0 = ti * TILE, n0 = tj * TILE;
+  auto issue = [&](int st) {
     const int kblk = (ka + st * SK) / SK;
-    char *buf = smem + (gs % NBUF) * STAGE;
+    char *buf = smem + (st % NBUF) * STAGE;
 #pragma unroll
     for (int j = 0; j < PERW; j++) {
       const int i = w + NW * j;
@@ -701,21 +680,20 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, in
       __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
     }
   };
-  const int ngs = nst * ntr;  // stages of the run
+  const int nst = (kb - ka) / SK;
 #pragma unroll
   for (int st = 0; st < NBUF - 1; st++)
-    if (st < ngs) issue(st);
-  for (int gs = 0; gs < ngs; gs++) {
-    const int s = (TPW == 1) ? gs : gs % nst;  // stage within the tile
-    // stages gs+1 .. min(ngs-1, gs+NBUF-2) may stay in flight
-    const int ahead = min(ngs - 1 - gs, NBUF - 2);
+    if (st < nst) issue(st);
+  for (int s = 0; s < nst; s++) {
+    // stages s+1 .. min(nst-1, s+NBUF-2) may stay in flight
+    const int ahead = min(nst - 1 - s, NBUF - 2);
     if (NBUF >= 5 && ahead >= 3) wait_vm<3 * PERW>();
     else if (NBUF >= 4 && ahead >= 2) wait_vm<2 * PERW>();
     else if (ahead >= 1) wait_vm<PERW>();
     else wait_vm<0>();
     __syncthreads();
-    if (gs + NBUF - 1 < ngs) issue(gs + NBUF - 1);
-    const char *cur = smem + (gs % NBUF) * STAGE;
+    if (s + NBUF - 1 < nst) issue(s + NBUF - 1);
+    const char *cur = smem + (s % NBUF) * STAGE;
     digit_stage<DT, NL, WI, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
     if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
 #pragma unroll
@@ -736,11 +714,9 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, in
 #pragma unroll
           for (int j = 0; j < WJ; j++) acc[l][i][j] = ACC{};
     }
-    if (s != nst - 1) continue;
-    if (TPW > 1) coords(first + gs / nst, bi, bj, z, tile);
-    const int m0 = bi * TILE, n0 = bj * TILE;
+  }
 
-  // epilogue of the tile: combine levels in FP64
+  // epilogue: combine levels in FP64
 #pragma unroll
   for (int i = 0; i < WI; i++)
 #pragma unroll
@@ -799,36 +775,18 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int first, in
       }
     }
   }
-    if (TPW > 1) {
-      // the run continues: clear the accumulators for the next tile
-#pragma unroll
-      for (int l = 0; l < NL; l++)
-#pragma unroll
-        for (int i = 0; i < WI; i++)
-#pragma unroll
-          for (int j = 0; j < WJ; j++) acc[l][i][j] = ACC{};
-#pragma unroll
-      for (int i = 0; i < WI; i++)
-#pragma unroll
-        for (int j = 0; j < WJ; j++)
-#pragma unroll
-          for (int v = 0; v < 16; v++) facc[i][j][v] = 0.0;
-    }
-  }
   // the next item's first loads overwrite the ring
   __syncthreads();
 }
 
-template <int DT, int S, int MODE, int NLB, int TILE, int TPW = 1>
+template <int DT, int S, int MODE, int NLB, int TILE>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, DigitCfg<DT, S, MODE, NLB, TILE>::NTHR)))
 __attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
                                    DigitCfg<DT, S, MODE, NLB, TILE>::WPE))) void digit_nt_kernel(
     oa_digit_args a, int total) {
-  // workgroup g takes the run of tiles [g*TPW, g*TPW + TPW)
   __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
-  const int runs = cdv(total, TPW);
-  const int g = single_item(runs);
-  if (g < runs) digit_body<DT, S, MODE, NLB, TILE, TPW>(a, g * TPW, total, smem);
+  const int item = single_item(total);
+  if (item < total) digit_body<DT, S, MODE, NLB, TILE>(a, item, smem);
 }
 
 template <int DT, int S, int MODE, int NLB, int TILE>
@@ -839,8 +797,7 @@ __attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
   __shared__ __attribute__((aligned(1024))) char smem[DigitCfg<DT, S, MODE, NLB, TILE>::SMEM];
   for_items(prefix[count], [&](int item) {
     const int i = group_find(prefix, count, item);
-    const int local = item - prefix[i];
-    digit_body<DT, S, MODE, NLB, TILE>(t[i], local, local + 1, smem);
+    digit_body<DT, S, MODE, NLB, TILE>(t[i], item - prefix[i], smem);
   });
 }
 
@@ -1139,10 +1096,8 @@ int run_backmul(hipStream_t s, const oa_split_plan &pl, int n, int K, const doub
   r.d1.alpha = alpha;
   if (!reuse) launch_slice_psi<DT, S>(s, pl, r, false);
   OA_LAUNCH_ITEMS((slice_rows_kernel<DT, NLB>), 256, r.items_rows, s, r.rows);
-  // runs of BM_RUN row tiles of one column tile per workgroup
-  const int runs = cdv(r.items_d1, BM_RUN);
-  hipLaunchKernelGGL((digit_nt_kernel<DT, S, 1, NLB, 128, BM_RUN>), dim3(cdv(runs, 8) * 8),
-                     dim3(DigitCfg<DT, S, 1, NLB, 128>::NTHR), 0, s, r.d1, r.items_d1);
+  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 1, NLB, 128>), (DigitCfg<DT, S, 1, NLB, 128>::NTHR),
+                  r.items_d1, s, r.d1);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
