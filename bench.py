@@ -277,8 +277,12 @@ def run_serving(args):
     sweep = []
     for rate in [float(r) for r in args.rates.split(",") if r.strip()]:
         specs = poisson_sessions(rate, args.duration, steps=args.serve_steps, seed=0)
-        ex = GpuExecutor(h, prec=prec, tol=args.tol, maxit=args.maxit)
+        # the batch workspace of all admitted sessions: at most ~60% of their
+        # footprints; sized once so re-plans never allocate
+        ex = GpuExecutor(h, prec=prec, tol=args.tol, maxit=args.maxit,
+                         reserve_bytes=min(budget // 4, 48 << 30))
         r = serve(specs, ex, budget, max_wall_s=args.duration + 120.0)
+        r.update({"replans": ex.replans, "plan_s": ex.t_plan, "exec_s": ex.t_exec})
         ex.release()
         torch.cuda.empty_cache()
         r["offered_req_s"] = rate

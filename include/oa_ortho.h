@@ -233,6 +233,9 @@ int oa_batch_destroy(oa_batch_t b);
 int oa_batch_plan(oa_batch_t b, int count, const oa_instance *inst, int prec);
 /* Bytes of device memory the planned batch holds (workspace + tables). */
 size_t oa_batch_footprint(oa_batch_t b);
+/* Pre-size the batch's workspace and table allocations, so that later plans
+ * within these sizes allocate nothing (a serving loop re-plans often). */
+int oa_batch_reserve(oa_batch_t b, size_t workspace_bytes, size_t table_bytes);
 /* Device bytes one instance of (n, npw) needs at `prec` (admission). */
 size_t oa_instance_footprint(int n, int npw, int prec);
 /* Enqueue one step of every planned instance on the handle's stream: no

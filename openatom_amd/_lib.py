@@ -116,6 +116,7 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_batch_destroy, _i, _vp)
     _sig(L.oa_batch_plan, _i, _vp, _i, ctypes.POINTER(oa_instance), _i)
     _sig(L.oa_batch_footprint, ctypes.c_size_t, _vp)
+    _sig(L.oa_batch_reserve, _i, _vp, ctypes.c_size_t, ctypes.c_size_t)
     _sig(L.oa_instance_footprint, ctypes.c_size_t, _i, _i, _i)
     _sig(L.oa_batch_run, _i, _vp)
     _sig(L.oa_batch_launch, _i, _vp)

@@ -119,13 +119,17 @@ void drop_graph(oa_batch *b) {
   b->graph = nullptr;
 }
 
+// Grow-only allocation that at least doubles, so a batch whose membership
+// keeps changing (the serving loop) reallocates rarely; oa_batch_reserve
+// sizes it once up front.
 int grow(oa_workspace &w, size_t bytes) {
   if (bytes <= w.bytes) return OA_SUCCESS;
   if (w.ptr) OA_HIP_CHECK(hipFree(w.ptr));
   w.ptr = nullptr;
+  const size_t want = std::max(bytes, 2 * w.bytes);
   w.bytes = 0;
   const size_t gran = (size_t)1 << 21;
-  const size_t sz = (bytes + gran - 1) / gran * gran;
+  const size_t sz = (want + gran - 1) / gran * gran;
   OA_HIP_CHECK(hipMalloc(&w.ptr, sz));
   w.bytes = sz;
   return OA_SUCCESS;
@@ -525,6 +529,24 @@ int oa_batch_plan(oa_batch_t b, int count, const oa_instance *inst, int prec) {
 }
 
 size_t oa_batch_footprint(oa_batch_t b) { return b ? b->ws.bytes + b->tab.bytes : 0; }
+
+int oa_batch_reserve(oa_batch_t b, size_t workspace_bytes, size_t table_bytes) {
+  if (!b) return 1;
+  DevGuard dg(b->c->device);
+  if (!dg.ok) return OA_ERR_NO_DEVICE;
+  OA_HIP_CHECK(hipStreamSynchronize(b->gstream));
+  OA_HIP_CHECK(hipStreamSynchronize(b->c->stream));
+  if (workspace_bytes > b->ws.bytes || table_bytes > b->tab.bytes) {
+    // the plan's addresses point into these allocations: plan again after
+    drop_graph(b);
+    int st = grow(b->ws, workspace_bytes);
+    if (st) return st;
+    st = grow(b->tab, table_bytes);
+    if (st) return st;
+    if (b->count > 0) return plan(b, b->count, b->inst.data(), b->prec);
+  }
+  return OA_SUCCESS;
+}
 
 size_t oa_instance_footprint(int n, int npw, int prec) {
   if (n < 1 || npw < 1) return 0;

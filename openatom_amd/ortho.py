@@ -267,6 +267,11 @@ class Batch:
     def footprint(self) -> int:
         return int(_lib.lib().oa_batch_footprint(self.b))
 
+    def reserve(self, workspace_bytes: int, table_bytes: int = 1 << 22) -> None:
+        """Size the workspace once, so re-plans within it allocate nothing."""
+        check(_lib.lib().oa_batch_reserve(self.b, workspace_bytes, table_bytes),
+              "oa_batch_reserve")
+
     def close(self) -> None:
         if getattr(self, "b", None):
             _lib.lib().oa_batch_destroy(self.b)

@@ -4,8 +4,9 @@ Model.  An MD trajectory (a session: one state block of n states and
 npw = 64 n plane waves) arrives, stays resident on the device for `steps`
 MD steps and leaves.  Each MD step issues one request: an ortho step (S,
 S->T, T*psi) of the session's current coefficients; when it completes, the
-client perturbs the orthonormalised block by eps (the simulated dynamics)
-and issues the next step.  Sessions arrive as a Poisson process; an offered
+client perturbs the orthonormalised block by eps (the simulated dynamics:
+psi <- T psi + (-1)^k eps P with an envelope-shaped block P drawn once per
+session, one fused pass) and issues the next step.  Sessions arrive as a Poisson process; an offered
 load of R requests/s is R / steps sessions/s.
 
 Scheduling.
@@ -135,6 +136,7 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
     pending = deque(sorted(specs, key=lambda s: s.arrival))
     waits, lat = [], []
     finished, iterations, batch_sizes = [], 0, []
+    t_step = t_advance = 0.0  # loop time in batch steps / client MD steps
     while pending or sched.waiting or sched.active:
         now = clock.now()
         if max_wall_s is not None and now > max_wall_s:
@@ -142,18 +144,26 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
         while pending and pending[0].arrival <= now:
             sched.arrive(pending.popleft())
         for s in sched.admit(now):
-            executor.open(s)
-            s.issued_at = clock.now()
-            waits.append(s.issued_at - s.spec.arrival)
-        batch = sched.batch()
+            executor.open(s)           # coefficients start moving to the device
+            s.issued_at = None
+        t_ready = clock.now()
+        for s in sched.active.values():
+            if s.issued_at is None and executor.ready(s):
+                s.issued_at = t_ready  # first request: coefficients resident
+                waits.append(s.issued_at - s.spec.arrival)
+        batch = [s for s in sched.batch() if s.issued_at is not None]
         if not batch:
-            if pending:
+            if sched.active:
+                clock.sleep_until(clock.now() + 1e-4)  # admitted sessions still loading
+            elif pending:
                 clock.sleep_until(pending[0].arrival)
             elif sched.waiting:
                 break  # nothing active and nothing admissible: cannot progress
             continue
+        t0 = clock.now()
         executor.step(batch)
         t_done = clock.now()
+        t_step += t_done - t0
         iterations += 1
         batch_sizes.append(len(batch))
         for s in batch:
@@ -167,6 +177,7 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
             else:
                 executor.advance(s)
         t_issue = clock.now()
+        t_advance += t_issue - t_done
         for s in batch:
             if s.done < s.spec.steps:
                 s.issued_at = t_issue
@@ -188,6 +199,8 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
         "admission_wait_p99_ms": None if not waits else 1e3 * pct(waits, 99),
         "iterations": iterations,
         "mean_batch": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
+        "batch_step_s": t_step,
+        "client_md_step_s": t_advance,
         "peak_footprint_bytes": sched.peak_used,
         "budget_bytes": sched.budget,
     }
@@ -199,57 +212,94 @@ class GpuExecutor:
 
     def __init__(self, handle=None, prec="int8x5", eps: float = 1e-3, nspin: int = 2,
                  tol: float = 1e-12, maxit: int = 12, use_graph: bool = True,
-                 record: bool = False):
+                 record: bool = False, reserve_bytes: int = 0):
         import torch
 
         from .ortho import Batch, default_handle, instance_footprint
         self.torch = torch
         self.h = handle or default_handle()
         self.batch = Batch(self.h)
+        if reserve_bytes:
+            self.batch.reserve(reserve_bytes)
         self.prec, self.eps, self.nspin, self.tol, self.maxit = prec, eps, nspin, tol, maxit
         self.use_graph = use_graph
         self._fp = instance_footprint
         self.members: tuple = ()
+        self.replans, self.t_plan, self.t_exec = 0, 0.0, 0.0
+        self.stable, self.graph_after = 0, 3
+        self.side = torch.cuda.Stream()  # default priority
         self.record = record
         self.records: dict[int, list] = {}
 
     def footprint(self, n: int, npw: int) -> int:
-        # psi and psi_out blocks, S and T, plus the batch workspace share
-        return 2 * 16 * n * npw + 2 * 8 * n * n + self._fp(n, npw, self.prec)
+        # psi, psi_out and the MD perturbation block, S and T, plus the batch
+        # workspace share
+        return 3 * 16 * n * npw + 2 * 8 * n * n + self._fp(n, npw, self.prec)
 
     def open(self, s: Session) -> None:
-        from .workload import make_states
+        """Make the session's coefficients resident on a side stream (nothing
+        in the generator synchronises the host), so the running batches are
+        not held up; ready() tells when they are there."""
+        from .workload import make_states, md_perturbation
         torch = self.torch
-        psi = make_states(s.spec.n, s.spec.npw, eps=1e-3, seed=s.spec.seed, device="cuda",
-                          rng="torch")
-        n = s.spec.n
-        s.state = {"psi": psi, "out": torch.empty_like(psi),
-                   "S": torch.empty((n, n), dtype=torch.float64, device="cuda"),
-                   "T": torch.empty((n, n), dtype=torch.float64, device="cuda")}
+        with torch.cuda.stream(self.side):
+            psi = make_states(s.spec.n, s.spec.npw, eps=1e-3, seed=s.spec.seed, device="cuda",
+                              rng="torch")
+            n = s.spec.n
+            s.state = {"psi": psi, "out": torch.empty_like(psi),
+                       "P": md_perturbation(n, s.spec.npw, seed=s.spec.seed * 1000 + 1),
+                       "S": torch.empty((n, n), dtype=torch.float64, device="cuda"),
+                       "T": torch.empty((n, n), dtype=torch.float64, device="cuda"),
+                       "ready": torch.cuda.Event()}
+            s.state["ready"].record(self.side)
+
+    def ready(self, s: Session) -> bool:
+        ev = s.state.get("ready")
+        if ev is None:
+            return True
+        if not ev.query():
+            return False
+        cur = self.torch.cuda.current_stream()
+        cur.wait_event(ev)
+        for t in ("psi", "out", "P", "S", "T"):
+            s.state[t].record_stream(cur)  # used from now on on the batch stream
+        s.state["ready"] = None
+        return True
 
     def step(self, batch: list[Session]) -> None:
         members = tuple(s.spec.sid for s in batch)
+        t0 = time.perf_counter()
+        self.stable = self.stable + 1 if members == self.members else 0
         if members != self.members:
             self.batch.plan([s.state["psi"] for s in batch], outs=[s.state["out"] for s in batch],
                             nkx0=1, nspin=self.nspin, tol=self.tol, maxit=self.maxit,
                             prec=self.prec, S=[s.state["S"] for s in batch],
                             T=[s.state["T"] for s in batch])
             self.members = members
+            self.replans += 1
+            t1 = time.perf_counter()
             self.batch.run()
-        elif self.use_graph:
+        elif self.use_graph and self.stable >= self.graph_after:
+            # a membership that held for a few steps is worth a captured graph
+            t1 = time.perf_counter()
             self.batch.launch()
         else:
+            t1 = time.perf_counter()
             self.batch.run()
-        self.torch.cuda.synchronize()
+        # wait for this batch only: sessions loading on the side stream go on
+        self.torch.cuda.current_stream().synchronize()
+        t2 = time.perf_counter()
+        self.t_plan += t1 - t0
+        self.t_exec += t2 - t1
         if self.record:
             for s in batch:
                 self.records.setdefault(s.spec.sid, []).append(s.state["out"].clone())
 
     def advance(self, s: Session) -> None:
-        from .workload import perturb_
+        """The client's MD step: psi <- T psi + (-1)^k eps P, one fused pass."""
+        from .workload import md_step_
         st = s.state
-        st["psi"].copy_(st["out"])
-        perturb_(st["psi"], self.eps, seed=s.spec.seed * 1000 + s.done)
+        md_step_(st["psi"], st["out"], st["P"], self.eps, s.done)
 
     def close(self, s: Session) -> None:
         s.state = None

@@ -38,8 +38,21 @@ def metric_weights(npw: int, nkx0: int = 1) -> np.ndarray:
     return w
 
 
+_DEV_CACHE: dict = {}
+
+
+def _on_device(key, make, device) -> torch.Tensor:
+    """Per-device cache of constant vectors (envelope, metric weights), so
+    repeated draws do not copy them from host memory each time."""
+    k = (key, str(torch.device(device)))
+    t = _DEV_CACHE.get(k)
+    if t is None:
+        t = _DEV_CACHE[k] = torch.from_numpy(make()).to(device)
+    return t
+
+
 def _complex_normal_block(rng, n: int, npw: int, device) -> torch.Tensor:
-    env = torch.from_numpy(np.repeat(envelope(npw), 2)).to(device)
+    env = _on_device(("env", npw), lambda: np.repeat(envelope(npw), 2), device)
     if isinstance(rng, np.random.Generator):
         Z = torch.from_numpy(rng.standard_normal((n, 2 * npw))).to(device)
     else:
@@ -50,9 +63,12 @@ def _complex_normal_block(rng, n: int, npw: int, device) -> torch.Tensor:
 
 
 def _cholqr2(Z: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+    # cholesky_ex: no host-side check of the factorisation's info flag, so
+    # generation on a side stream never stalls the host (S is a Gram matrix
+    # of a full-rank random block, positive definite by construction)
     for _ in range(2):
         S = (Z * w) @ Z.T
-        L = torch.linalg.cholesky(S)
+        L, _info = torch.linalg.cholesky_ex(S)
         eye = torch.eye(L.shape[0], dtype=L.dtype, device=L.device)
         Linv = torch.linalg.solve_triangular(L, eye, upper=False)
         Z = Linv @ Z
@@ -84,6 +100,25 @@ def make_states(n: int, npw: int, eps: float = 1e-3, seed: int = 0, nkx0: int = 
     return Q.contiguous()
 
 
+def md_perturbation(n: int, npw: int, seed: int, nkx0: int = 1,
+                    device: str | torch.device = "cuda") -> torch.Tensor:
+    """An envelope-shaped complex normal block with rows of unit metric norm,
+    drawn once per trajectory; the serving loop's simulated MD step is
+    psi_{k+1} = (T psi)_k + (-1)^k eps P (one fused pass over the block)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    P = _complex_normal_block(gen, n, npw, device)
+    w = _on_device(("w", npw, nkx0), lambda: metric_weights(npw, nkx0), device)
+    P /= torch.sqrt(((P * w) * P).sum(dim=1, keepdim=True))
+    return P
+
+
+def md_step_(psi: torch.Tensor, out: torch.Tensor, P: torch.Tensor, eps: float,
+             step: int) -> torch.Tensor:
+    """psi <- out + (-1)^step eps P, in place."""
+    return torch.add(out, P, alpha=(eps if step % 2 == 0 else -eps), out=psi)
+
+
 def perturb_(psi: torch.Tensor, eps: float, seed: int, nkx0: int = 1) -> torch.Tensor:
     """One simulated MD step on a resident block, in place: psi += eps times
     an envelope-shaped complex normal block (each row of metric norm eps),
@@ -93,7 +128,7 @@ def perturb_(psi: torch.Tensor, eps: float, seed: int, nkx0: int = 1) -> torch.T
     gen = torch.Generator(device=psi.device)
     gen.manual_seed(seed)
     P = _complex_normal_block(gen, n, npw, psi.device)
-    w = torch.from_numpy(metric_weights(npw, nkx0)).to(psi.device)
+    w = _on_device(("w", npw, nkx0), lambda: metric_weights(npw, nkx0), psi.device)
     nrm = torch.sqrt(((P * w) * P).sum(dim=1, keepdim=True))
     psi += P * (eps / nrm)
     return psi

@@ -7,7 +7,7 @@ import torch
 pytestmark = pytest.mark.gpu
 
 from openatom_amd.serve import GpuExecutor, SessionSpec, serve  # noqa: E402
-from openatom_amd.workload import make_states, perturb_  # noqa: E402
+from openatom_amd.workload import make_states, md_perturbation, md_step_  # noqa: E402
 
 
 def test_results_same_alone_and_in_mix(handle):
@@ -25,13 +25,14 @@ def test_results_same_alone_and_in_mix(handle):
     assert r["mean_batch"] > 1.0  # sessions overlapped on the device
     for sp in specs:
         psi = make_states(sp.n, sp.npw, eps=1e-3, seed=sp.seed, device="cuda", rng="torch")
+        P = md_perturbation(sp.n, sp.npw, seed=sp.seed * 1000 + 1)
         for k in range(steps):
             S, T, out, it, err = handle.ortho_step(psi, nkx0=1, nspin=2, tol=1e-12, maxit=12,
                                                    prec="int8x5")
             torch.cuda.synchronize()
             assert torch.equal(out, ex.records[sp.sid][k]), (sp.sid, k)
-            psi = out.clone()
-            perturb_(psi, 1e-3, seed=sp.seed * 1000 + k + 1)
+            psi = torch.empty_like(out)
+            md_step_(psi, out, P, 1e-3, k + 1)
 
 
 def test_serving_results_stay_orthonormal(handle):

@@ -30,6 +30,9 @@ class SimExecutor:
         self.opened.append(s.spec.sid)
         s.state = True
 
+    def ready(self, s):
+        return True
+
     def step(self, batch):
         self.batches.append([s.spec.sid for s in batch])
         self.clock.t += self.per_item * sum(s.spec.n for s in batch) / 128
