@@ -179,6 +179,34 @@ int oa_backmul(oa_handle_t h, int n, int npw, const double *M, int transM,
                const double *psi, long ldpsi, double alpha, double beta,
                double *out, long ldout, int prec);
 
+/* GSpace side of the Lambda round trip, fused into the backward multiply:
+ *   force[y][c] := beta * force[y][c] + alpha * w_c * (op(M) psi)[y][c],
+ * w_c = wkx0 for the leading 2*nkx0 doubles of every row (the kx = 0
+ * points) and 1 elsewhere, op as in oa_backmul.  It replaces the backward
+ * GEMM followed by the GSpace accumulation of its result:
+ *   minimisation, partialResultMsg (CP_State_GSpacePlane.C:2168-2195):
+ *     force -= w data, w = 1/2 at kx = 0 (double pack), 1/2 everywhere
+ *     (single pack)  ->  alpha = -1, wkx0 = 1/2, beta = 1 (or alpha = -1/2,
+ *     wkx0 = 1);
+ *   minimisation, CkReductionMsg (:2068-2106): w = 1/ov at kx = 0, 2/ov
+ *     elsewhere, ov = 2 (lda) or 1 (lsda)  ->  alpha = -2/ov, wkx0 = 1/2;
+ *   dynamics (:2180-2183 with doLambda :2247-2252): force = -data, then the
+ *     kx = 0 points scaled by 1/sqrt2  ->  alpha = -1, wkx0 = 1/sqrt2,
+ *     beta = 0 for the first contribution, 1 after.
+ * Returns the argument index on a bad argument (nkx0 outside [0, npw] is 8). */
+int oa_lambda_force(oa_handle_t h, int n, int npw, const double *M, int transM,
+                    const double *psi, long ldpsi, int nkx0, double wkx0,
+                    double alpha, double beta, double *force, long ldf,
+                    int prec);
+
+/* X[r][c] *= s for c < 2*nkx0 (every row r < n): the GSpace rescaling of the
+ * kx = 0 points -- psi by 1/sqrt2 and the force by sqrt2 before the dynamics
+ * Lambda forward (CP_State_GSpacePlane.C:1926-1947; nkx0 = npw scales every
+ * point, the single-pack form), the force by 1/sqrt2 in doLambda
+ * (:2247-2252). */
+int oa_scale_kx0(oa_handle_t h, int n, int npw, double *X, long ld, int nkx0,
+                 double s);
+
 /* One symmetric orthonormalisation step -- overlap, S->T, T*psi -- as the
  * reference's GSP -> PC -> Ortho -> PC -> GSP round trip performs it
  * (call stack: gspace.ci:158 -> CP_State_GSpacePlane.C:2975 ->

@@ -107,6 +107,8 @@ def lib() -> ctypes.CDLL:
          ctypes.POINTER(_d))
     _sig(L.oa_s_to_t_async, _i, _vp, _i, _vp, _i, _d, _i, _vp, _vp)
     _sig(L.oa_backmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
+    _sig(L.oa_lambda_force, _i, _vp, _i, _i, _vp, _i, _vp, _l, _i, _d, _d, _d, _vp, _l, _i)
+    _sig(L.oa_scale_kx0, _i, _vp, _i, _i, _vp, _l, _i, _d)
     _sig(L.oa_ortho_step, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp, _l,
          ctypes.POINTER(_i), ctypes.POINTER(_d))
     _dp = ctypes.POINTER(ctypes.c_double)

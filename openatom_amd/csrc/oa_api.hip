@@ -451,22 +451,19 @@ int oa_s_to_t_async(oa_handle_t c, int n, const double *S, int nspin, double tol
   return oa_launch_s_to_t(c->stream, n, S, nspin, tol, maxit, T, report, c->ns.ptr);
 }
 
-int oa_backmul(oa_handle_t c, int n, int npw, const double *M, int transM, const double *psi,
-               long ldpsi, double alpha, double beta, double *out, long ldout, int prec) {
-  if (!c) return OA_ERR_NO_DEVICE;
-  if (n < 0) return 2;
-  if (npw < 0) return 3;
-  if (n > 0 && !M) return 4;
-  if (ldpsi < 2L * npw) return 7;
-  if (ldout < 2L * npw) return 11;
-  if (n == 0 || npw == 0) return OA_SUCCESS;
+}  // extern "C"
+
+// out = beta out + alpha w .* op(M) psi, w = 1 except columns < cw (alpha_w).
+static int backmul_cols(oa_context *c, int n, int npw, const double *M, int transM,
+                        const double *psi, long ldpsi, double alpha, double beta, double *out,
+                        long ldout, int prec, int cw, double alpha_w) {
   DeviceGuard g(c->device);
   const int K = 2 * npw;
   if ((prec & 0xff) != OA_PREC_FP64) {
     int st = oa_ws_require(c, oa_split_ws_bytes(prec, n, K));
     if (st) return st;
     return oa_launch_backmul_split(c->stream, prec, n, K, M, transM, psi, ldpsi, 0, alpha, beta,
-                                   out, ldout, c->ws.ptr, c->ws.bytes);
+                                   out, ldout, c->ws.ptr, c->ws.bytes, cw, alpha_w);
   }
   oa_gemm_problem p;
   p.M = n;
@@ -481,7 +478,52 @@ int oa_backmul(oa_handle_t c, int n, int npw, const double *M, int transM, const
   p.alpha = alpha;
   p.beta = beta;
   p.diag_add = 0.0;
+  p.cw = cw;
+  p.alpha_w = alpha_w;
   return oa_launch_gemm_f64(c->stream, transM != 0, false, &p, 1, nullptr);
+}
+
+extern "C" {
+
+int oa_backmul(oa_handle_t c, int n, int npw, const double *M, int transM, const double *psi,
+               long ldpsi, double alpha, double beta, double *out, long ldout, int prec) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (n > 0 && !M) return 4;
+  if (ldpsi < 2L * npw) return 7;
+  if (ldout < 2L * npw) return 11;
+  if (n == 0 || npw == 0) return OA_SUCCESS;
+  return backmul_cols(c, n, npw, M, transM, psi, ldpsi, alpha, beta, out, ldout, prec, 0, 0.0);
+}
+
+int oa_lambda_force(oa_handle_t c, int n, int npw, const double *M, int transM,
+                    const double *psi, long ldpsi, int nkx0, double wkx0, double alpha,
+                    double beta, double *force, long ldf, int prec) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (n > 0 && !M) return 4;
+  if (n > 0 && npw > 0 && !psi) return 6;
+  if (ldpsi < 2L * npw) return 7;
+  if (nkx0 < 0 || nkx0 > npw) return 8;
+  if (n > 0 && npw > 0 && !force) return 12;
+  if (ldf < 2L * npw) return 13;
+  if (n == 0 || npw == 0) return OA_SUCCESS;
+  return backmul_cols(c, n, npw, M, transM, psi, ldpsi, alpha, beta, force, ldf, prec,
+                      2 * nkx0, alpha * wkx0);
+}
+
+int oa_scale_kx0(oa_handle_t c, int n, int npw, double *X, long ld, int nkx0, double s) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 0) return 2;
+  if (npw < 0) return 3;
+  if (n > 0 && nkx0 > 0 && !X) return 4;
+  if (ld < 2L * npw) return 5;
+  if (nkx0 < 0 || nkx0 > npw) return 6;
+  if (n == 0 || nkx0 == 0) return OA_SUCCESS;
+  DeviceGuard g(c->device);
+  return oa_launch_scale_cols(c->stream, n, 2 * nkx0, X, ld, s);
 }
 
 int oa_ortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, int nkx0,

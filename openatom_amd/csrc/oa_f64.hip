@@ -184,6 +184,8 @@ struct ProbRegs {
   double *C;
   long ldc;
   double alpha, beta, diag_add;
+  int cw;
+  double alpha_w;
 };
 
 __device__ __forceinline__ ProbRegs pick_problem(const oa_gemm_problem &p0,
@@ -202,6 +204,8 @@ __device__ __forceinline__ ProbRegs pick_problem(const oa_gemm_problem &p0,
   r.alpha = y ? p1.alpha : p0.alpha;
   r.beta = y ? p1.beta : p0.beta;
   r.diag_add = y ? p1.diag_add : p0.diag_add;
+  r.cw = y ? p1.cw : p0.cw;
+  r.alpha_w = y ? p1.alpha_w : p0.alpha_w;
   return r;
 }
 
@@ -237,7 +241,8 @@ __device__ __forceinline__ void gemm_body(const ProbRegs &P, int logical, double
 #pragma unroll
         for (int j = 0; j < WM; j++) {
           const int n = n0 + yrow0 + j * 16 + r;
-          if (n < P.N) crow[n] = alpha * acc[i][j][v] + ((m == n) ? dadd : 0.0);
+          const double al = (n < P.cw) ? P.alpha_w : alpha;
+          if (n < P.N) crow[n] = al * acc[i][j][v] + ((m == n) ? dadd : 0.0);
         }
       }
   } else {
@@ -251,8 +256,9 @@ __device__ __forceinline__ void gemm_body(const ProbRegs &P, int logical, double
 #pragma unroll
         for (int j = 0; j < WM; j++) {
           const int n = n0 + yrow0 + j * 16 + r;
+          const double al = (n < P.cw) ? P.alpha_w : alpha;
           if (n < P.N)
-            crow[n] = alpha * acc[i][j][v] + beta * crow[n] + ((m == n) ? dadd : 0.0);
+            crow[n] = al * acc[i][j][v] + beta * crow[n] + ((m == n) ? dadd : 0.0);
         }
       }
   }
@@ -308,6 +314,7 @@ void gemm_f64_grouped(const oa_gemm_item *__restrict__ t, const int *__restrict_
     P.X = it.p.X; P.ldx = it.p.ldx; P.Y = it.p.Y; P.ldy = it.p.ldy;
     P.C = it.p.C; P.ldc = it.p.ldc;
     P.alpha = it.p.alpha; P.beta = it.p.beta; P.diag_add = it.p.diag_add;
+    P.cw = it.p.cw; P.alpha_w = it.p.alpha_w;
     gemm_body<false, false, WM>(P, item - prefix[i], smem);
     __syncthreads();
   });
@@ -661,7 +668,28 @@ OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym, int nz_override = 0
   return pl;
 }
 
+// X[r][c] *= s over the leading ncols columns of each row: one thread per
+// element, rows strided over the grid.
+__global__ void scale_cols_kernel(int rows, int ncols, double *X, long ld, double scale) {
+  const long total = (long)rows * ncols;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long)gridDim.x * blockDim.x) {
+    double *p = X + (i / ncols) * ld + (i % ncols);
+    *p *= scale;
+  }
+}
+
 }  // namespace
+
+int oa_launch_scale_cols(hipStream_t s, int rows, int ncols, double *X, long ld, double scale) {
+  const long total = (long)rows * ncols;
+  if (total <= 0) return OA_SUCCESS;
+  long blocks = (total + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rows, ncols, X,
+                     ld, scale);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
 
 int oa_launch_gemm_f64(hipStream_t s, bool tx, bool ty, const oa_gemm_problem *p, int nprob,
                        const int32_t *skip) {

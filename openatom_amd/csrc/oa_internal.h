@@ -51,6 +51,10 @@ struct oa_gemm_problem {
   double *C;
   long ldc;
   double alpha, beta, diag_add;
+  // columns n < cw of C use alpha_w instead of alpha (the kx = 0 weight of
+  // the GSpace force update); cw = 0 leaves every column at alpha
+  int cw = 0;
+  double alpha_w = 0.0;
 };
 
 // Launch one or two independent problems of the same shape class in one grid
@@ -62,6 +66,9 @@ int oa_launch_gemm_f64(hipStream_t s, bool tx, bool ty, const oa_gemm_problem *p
 // Symmetric overlap, FP64 MFMA, upper-triangle tiles, split-K.
 // S (row-major n x n, full) = alpha * sum_k w_k psi[x][k] psi[y][k] with
 // w_k = 1/2 for k < kx0_end and 1 otherwise.
+// X[r][c] *= s for r < rows, c < ncols (row-major, leading dimension ld).
+int oa_launch_scale_cols(hipStream_t s, int rows, int ncols, double *X, long ld, double scale);
+
 size_t oa_overlap_f64_ws_bytes(int n, int K);
 int oa_launch_overlap_f64(hipStream_t s, int n, int K, const double *psi,
                           long ld, int kx0_end, double alpha, double *S,
@@ -110,7 +117,8 @@ int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K,
                             const double *M, int transM, const double *psi,
                             long ldpsi, int kx0_end, double alpha, double beta,
-                            double *out, long ldout, void *ws, size_t ws_bytes);
+                            double *out, long ldout, void *ws, size_t ws_bytes,
+                            int cw = 0, double alpha_w = 0.0);
 
 // ---------------------------------------------------------------- grouped
 // A grouped launch walks `total` work items spread over `count` instances:
@@ -162,7 +170,9 @@ struct oa_digit_args {
   long dbs;                          // DB slice stride (elements)
   int np, Kp;
   const unsigned long long *colmax;  // per chunk of output columns
-  int kchunk, pad;
+  int kchunk;
+  int cw;                            // MODE 1: columns < cw use alpha_w
+  double alpha_w;
 };
 struct oa_sreduce_args {
   const double *part;

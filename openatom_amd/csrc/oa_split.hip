@@ -767,8 +767,9 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       } else {
         const int m = m0 + lm;
         if (m < a.M && nn < a.N) {
-          const double r = a.alpha * (a.diagv[m] * ldexp(pd[v], ez - 7) +
-                                      ldexp(val, a.fexp[m] + ez - 14));
+          const double al = (nn < a.cw) ? a.alpha_w : a.alpha;
+          const double r = al * (a.diagv[m] * ldexp(pd[v], ez - 7) +
+                                 ldexp(val, a.fexp[m] + ez - 14));
           double *o = a.out + (size_t)m * a.ldo + nn;
           *o = (a.beta != 0.0) ? r + a.beta * (*o) : r;
         }
@@ -1089,11 +1090,13 @@ int run_overlap(hipStream_t s, const oa_split_plan &pl, int n, int K, const doub
 template <int DT, int S, int NLB>
 int run_backmul(hipStream_t s, const oa_split_plan &pl, int n, int K, const double *M, int transM,
                 const double *psi, long ldpsi, int kx0_end, bool reuse, double alpha,
-                double beta, double *out, long ldout, char *ws) {
+                double beta, double *out, long ldout, char *ws, int cw, double alpha_w) {
   oa_split_inst_args r;
   fill_inst(pl, n, K, psi, ldpsi, kx0_end, 1.0, nullptr, M, transM, beta, out, ldout, ws,
             nullptr, nullptr, &r);
   r.d1.alpha = alpha;
+  r.d1.cw = cw;
+  r.d1.alpha_w = alpha_w;
   if (!reuse) launch_slice_psi<DT, S>(s, pl, r, false);
   OA_LAUNCH_ITEMS((slice_rows_kernel<DT, NLB>), 256, r.items_rows, s, r.rows);
   OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 1, NLB, 128>), (DigitCfg<DT, S, 1, NLB, 128>::NTHR),
@@ -1201,10 +1204,11 @@ int dispatch_overlap(hipStream_t st, const oa_split_plan &pl, int n, int K, cons
 template <int DT>
 int dispatch_backmul(hipStream_t st, const oa_split_plan &pl, bool lean, int n, int K,
                      const double *M, int transM, const double *psi, long ldpsi, int kx0_end,
-                     bool reuse, double alpha, double beta, double *out, long ldout, char *ws) {
+                     bool reuse, double alpha, double beta, double *out, long ldout, char *ws,
+                     int cw, double alpha_w) {
 #define OA_BM(SS, NL)                                                                        \
   return run_backmul<DT, SS, NL>(st, pl, n, K, M, transM, psi, ldpsi, kx0_end, reuse, alpha, \
-                                 beta, out, ldout, ws);
+                                 beta, out, ldout, ws, cw, alpha_w);
   switch (pl.S) {
     case 2: OA_BM(2, 1)
     case 3: if (lean) { OA_BM(3, 1) } else { OA_BM(3, 2) }
@@ -1343,7 +1347,8 @@ int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *
 
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double *M, int transM,
                             const double *psi, long ldpsi, int kx0_end, double alpha, double beta,
-                            double *out, long ldout, void *ws, size_t ws_bytes) {
+                            double *out, long ldout, void *ws, size_t ws_bytes, int cw,
+                            double alpha_w) {
   int dt, Sl;
   if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
   const oa_split_plan pl = make_plan(prec & 0xffff, n, K, 128);
@@ -1353,13 +1358,13 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double 
   char *w = (char *)ws;
   if (dt == DT_I8)
     return dispatch_backmul<DT_I8>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
-                                   alpha, beta, out, ldout, w);
+                                   alpha, beta, out, ldout, w, cw, alpha_w);
   if (dt == DT_F16)
     return dispatch_backmul<DT_F16>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
-                                    alpha, beta, out, ldout, w);
+                                    alpha, beta, out, ldout, w, cw, alpha_w);
   if (dt == DT_F32)
     return dispatch_backmul<DT_F32>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
-                                    alpha, beta, out, ldout, w);
+                                    alpha, beta, out, ldout, w, cw, alpha_w);
   return dispatch_backmul<DT_BF16>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,
-                                   alpha, beta, out, ldout, w);
+                                   alpha, beta, out, ldout, w, cw, alpha_w);
 }

@@ -21,6 +21,7 @@ library runs every product on the GPU.  Names follow the reference:
 from __future__ import annotations
 
 import ctypes
+import math
 
 import torch
 
@@ -165,6 +166,28 @@ class Handle:
                                     _ld(psi), alpha, beta, _ptr(out), _ld(out), pc),
               "oa_backmul")
         return out
+
+    def lambda_force(self, M: torch.Tensor, psi: torch.Tensor, force: torch.Tensor,
+                     transM: bool = False, nkx0: int = 1, wkx0: float = 0.5,
+                     alpha: float = -1.0, beta: float = 1.0, prec="fp64") -> torch.Tensor:
+        """force := beta force + alpha w .* op(M) psi in place, w = wkx0 on the
+        leading 2*nkx0 doubles of each row and 1 elsewhere."""
+        n = psi.shape[0]
+        _need(psi, "psi")
+        _need(M, "M", n)
+        _need(force, "force", n)
+        npw = psi.shape[1] // 2
+        check(_lib.lib().oa_lambda_force(self.h, n, npw, _ptr(M), 1 if transM else 0, _ptr(psi),
+                                         _ld(psi), nkx0, wkx0, alpha, beta, _ptr(force),
+                                         _ld(force), prec_code(prec)), "oa_lambda_force")
+        return force
+
+    def scale_kx0(self, X: torch.Tensor, nkx0: int, s: float) -> torch.Tensor:
+        """X[:, :2*nkx0] *= s in place."""
+        _need(X, "X")
+        check(_lib.lib().oa_scale_kx0(self.h, X.shape[0], X.shape[1] // 2, _ptr(X), _ld(X),
+                                      nkx0, s), "oa_scale_kx0")
+        return X
 
     def ortho_step(self, psi: torch.Tensor, nkx0: int = 1, nspin: int = 1,
                    tol: float = 1e-15, maxit: int = 1000, prec="fp64",
@@ -399,3 +422,83 @@ class Ortho:
         st = self.handle.dgemm("N", "N", n, n, n, 1.0, Lx, n, orthoT, n, 0.0, G, n)
         check(st, "oa_dgemm")
         return G
+
+
+RAD2I = 1.0 / math.sqrt(2.0)  # as the reference computes it (CP_State_GSpacePlane.C:1929)
+
+
+class GSpacePlane:
+    """GSpace side of the Lambda round trip for a whole state block
+    (CP_State_GSpacePlane.C).  The accumulation of the PairCalculator's
+    backward result into the force runs fused into the backward multiply
+    (oa_lambda_force), so the result block the reference ships back
+    (partialResultMsg / CkReductionMsg) is never materialised.
+
+    cp_min_opt = 1: minimisation, 0: dynamics.  double_pack: gamma-point
+    storage, where the leading nkx0 points are the kx = 0 plane."""
+
+    def __init__(self, nkx0: int = 1, double_pack: bool = True, cp_min_opt: int = 1,
+                 nspin: int = 1, prec="fp64", handle: Handle | None = None):
+        self.nkx0 = nkx0 if double_pack else 0
+        self.double_pack = double_pack
+        self.cp_min_opt = cp_min_opt
+        self.lsda = nspin != 1
+        self.prec = prec
+        self.handle = handle or default_handle()
+
+    def _kx0_points(self, X: torch.Tensor) -> int:
+        # single pack scales every point (CP_State_GSpacePlane.C:1942-1946)
+        return self.nkx0 if self.double_pack else X.shape[1] // 2
+
+    def scaleForLambda(self, psi: torch.Tensor, force: torch.Tensor) -> None:
+        """Dynamics, before the Lambda forward (CP_State_GSpacePlane.C:
+        1926-1947): psi *= 1/sqrt2 and force *= sqrt2 on the kx = 0 points
+        (double pack) or on every point (single pack), in place."""
+        if self.cp_min_opt != 0:
+            return
+        k = self._kx0_points(psi)
+        self.handle.scale_kx0(psi, k, RAD2I)
+        self.handle.scale_kx0(force, k, math.sqrt(2.0))
+
+    def unpackLambda(self, L: torch.Tensor, psi: torch.Tensor, force: torch.Tensor,
+                     reduction: bool = False) -> torch.Tensor:
+        """Minimisation: force -= w .* (Psi Lambda), the asymmetric backward
+        product (ckPairCalculator.C:1842, out_j = sum_i psi_i L[i][j])
+        accumulated with the GSpace weights.
+          partialResultMsg (CP_State_GSpacePlane.C:2168-2195): w = 1/2 at
+            kx = 0 and 1 elsewhere (double pack); 1/2 everywhere (single pack).
+          CkReductionMsg (:2068-2106): w = 1/ov at kx = 0, 2/ov elsewhere
+            (double pack), 1/ov everywhere (single pack); ov = 2 (lda) or 1
+            (lsda).
+        In place; returns force."""
+        if self.cp_min_opt != 1:
+            raise ValueError("unpackLambda accumulates minimisation forces; dynamics uses "
+                             "unpackGamma")
+        if reduction:
+            ov = 1.0 if self.lsda else 2.0
+            alpha, wk = (-2.0 / ov, 0.5) if self.double_pack else (-1.0 / ov, 1.0)
+        else:
+            alpha, wk = (-1.0, 0.5) if self.double_pack else (-0.5, 1.0)
+        return self.handle.lambda_force(L, psi, force, transM=True, nkx0=self.nkx0, wkx0=wk,
+                                        alpha=alpha, beta=1.0, prec=self.prec)
+
+    def unpackGamma(self, gamma: torch.Tensor, orthoT: torch.Tensor, psi: torch.Tensor,
+                    F: torch.Tensor, force: torch.Tensor) -> torch.Tensor:
+        """Dynamics: the PairCalculator returns data = Psi Gamma^T - F T
+        (multiplyResult with Gamma accumulated onto bwMultiplyDynOrthoT's
+        -F T, ckPairCalculator.C:1965-2031, :1312-1316); the GSpace sets
+        force = -data (CP_State_GSpacePlane.C:2177-2183) and doLambda
+        scales its kx = 0 points by 1/sqrt2 (:2240-2245).  psi and F are the
+        blocks the PairCalculator received (after scaleForLambda); force
+        must not share memory with F.  Returns force."""
+        if self.cp_min_opt != 0:
+            raise ValueError("unpackGamma is the dynamics path")
+        if force.untyped_storage().data_ptr() == F.untyped_storage().data_ptr():
+            raise ValueError("force must not alias F: F T is read while force is written")
+        wk = RAD2I if self.nkx0 > 0 else 1.0
+        h = self.handle
+        h.lambda_force(gamma, psi, force, transM=True, nkx0=self.nkx0, wkx0=wk, alpha=-1.0,
+                       beta=0.0, prec=self.prec)
+        # -F T as bwMultiplyDynOrthoT forms it: DGEMM('N','N') with T as received
+        return h.lambda_force(orthoT, F, force, transM=False, nkx0=self.nkx0, wkx0=wk,
+                              alpha=1.0, beta=1.0, prec=self.prec)

@@ -44,6 +44,14 @@
  *                        returns and the nine loop nests of op(A) x op(B) in
  *                        {N, T, C}^2 with their summation order.  Pinned
  *                        bitwise against the compiled reference zgemm_.
+ *   oao_gsp_unpack       the GSpace accumulation of the backward result into
+ *                        the force: unpackLambda(partialResultMsg)
+ *                        CP_State_GSpacePlane.C:2168-2195, unpackLambda(
+ *                        CkReductionMsg) :2068-2106, and for dynamics
+ *                        force = -data (:2180-2183) with doLambda's kx = 0
+ *                        rescaling (:2240-2245).  Pinned by the constraint it
+ *                        enforces: for states with S = 2I (lda) or I (lsda)
+ *                        the updated force has zero Lambda with them.
  *
  * Every matrix of states is row-major [n][K] doubles with K = 2*npw (complex
  * coefficients interleaved re,im), exactly the collated layout the reference
@@ -509,6 +517,46 @@ int oao_ortho_step(int n, int npw, const double *psi, int nkx0, int nspin,
     for (int k = 0; k < 2 * nkx0; k++) psi_out[(size_t)i * K + k] *= rad2;
   free(sc);
   return st;
+}
+
+/* GSpace accumulation of a PairCalculator backward result `data` ([n][2npw])
+ * into `force` (same layout).  The kx = 0 points are the leading nkx0
+ * complex points of every row.
+ *   mode 0, minimisation, partialResultMsg (CP_State_GSpacePlane.C:
+ *     2168-2195): double pack: force -= wght*data, wght = 0.5 on kx = 0
+ *     points, 1 elsewhere; single pack: force -= 0.5*data.
+ *   mode 1, minimisation, CkReductionMsg (:2068-2106): overlap = 2 (lda) or
+ *     1 (lsda); double pack: wght = 1/overlap on kx = 0 points, 2/overlap
+ *     elsewhere; single pack: wght = 1/overlap.
+ *   mode 2, dynamics: force = -data (:2180-2183, one contribution), then
+ *     doLambda multiplies the kx = 0 points by 1/sqrt(2) (:2240-2245;
+ *     double pack only: the single-pack layout has no kx = 0 block). */
+void oao_gsp_unpack(int n, int npw, const double *data, int nkx0, int double_pack,
+                    int mode, int lsda, double *force) {
+  const size_t K = 2 * (size_t)npw;
+  const double overlap = lsda ? 1.0 : 2.0;
+  const double rad2i = 1.0 / sqrt(2.0);
+  for (int r = 0; r < n; r++) {
+    const double *d = data + r * K;
+    double *f = force + r * K;
+    for (int k = 0; k < npw; k++) {
+      const int kx0 = double_pack && k < nkx0;
+      if (mode == 2) {
+        for (int c = 0; c < 2; c++) {
+          f[2 * k + c] = d[2 * k + c] * (-1.0);
+          if (kx0) f[2 * k + c] *= rad2i;
+        }
+        continue;
+      }
+      double wght;
+      if (mode == 0)
+        wght = double_pack ? (kx0 ? 0.5 : 1.0) : 0.5;
+      else
+        wght = double_pack ? (kx0 ? 1.0 / overlap : 2.0 / overlap) : 1.0 / overlap;
+      f[2 * k] -= wght * d[2 * k];
+      f[2 * k + 1] -= wght * d[2 * k + 1];
+    }
+  }
 }
 
 /* max |S(psi) - occ*I| with S in the 2Re - G0 metric of oao_overlap. */

@@ -84,6 +84,8 @@ def lib() -> ctypes.CDLL:
                                      ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
                                      ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.oao_zgemm_info.restype = ctypes.c_int
+        L.oao_gsp_unpack.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -222,6 +224,18 @@ def ortho_step(psi: np.ndarray, nkx0: int = 1, nspin: int = 1, tol: float = 1e-1
                               _p(T), _p(out), ctypes.byref(it), ctypes.byref(err),
                               nplanes, gemm_ptr(gemm))
     return dict(S=S, T=T, psi_out=out, iters=it.value, err=err.value, status=st)
+
+
+def gsp_unpack(data: np.ndarray, force: np.ndarray, nkx0: int = 1, double_pack: bool = True,
+               mode: int = 0, lsda: bool = False) -> np.ndarray:
+    """GSpace accumulation of a backward result into the force, in place
+    (mode 0 partialResultMsg, 1 CkReductionMsg, 2 dynamics)."""
+    n, K = data.shape
+    assert force.shape == data.shape and force.flags.c_contiguous
+    data = np.ascontiguousarray(data)
+    lib().oao_gsp_unpack(n, K // 2, _p(data), nkx0, 1 if double_pack else 0, mode,
+                         1 if lsda else 0, _p(force))
+    return force
 
 
 def orthonormality_residual(psi: np.ndarray, nkx0: int = 1, occ: float = 1.0,

@@ -339,3 +339,65 @@ def test_gwbse_states_overlap(oracle, kpt):
     assert np.array_equal(S.view(np.float64), d["S_cm"].view(np.float64))
     assert np.abs(S.T - np.eye(nst)).max() < 5e-15
     assert np.abs(S - S.conj().T).max() < 1e-16  # Hermitian
+
+
+# ---- GSpace accumulation of the Lambda round trip --------------------------
+#
+# The pin is the constraint the accumulation enforces.  Lambda is the plain
+# dot product L[x][y] = sum_d psi_x[d] F_y[d] over the stored doubles, and the
+# stored force carries the two members of a +-G pair in one kx > 0 point.  For
+# states orthonormal in the path metric (S = 2I under lda, S = I under lsda,
+# Lambda doubled by Ortho under lsda) the weights 1 (kx > 0) and 1/2 (kx = 0)
+# make the updated force Lambda-free: L(psi, F') = 0.  A wrong weight, a
+# dropped kx = 0 branch or a transposed Lambda leaves O(|L|) behind.
+
+def _gsp_states(n, npw, nkx0, double_pack, lsda, seed):
+    # single pack: every point has weight 1 in the metric (no +-G pairing)
+    psi = make_states(n, npw, eps=0.0, seed=seed, nkx0=nkx0 if double_pack else npw).numpy()
+    return psi * (1.0 if lsda else np.sqrt(2.0))
+
+
+@pytest.mark.parametrize("mode,lsda,double_pack", [
+    (0, False, True), (0, True, True), (0, False, False), (0, True, False),
+    (1, False, True), (1, True, True), (1, False, False), (1, True, False)])
+def test_gsp_unpack_minimisation_projects_out_lambda(oracle, mode, lsda, double_pack):
+    n, npw, nkx0 = 10, 257, 7
+    psi = _gsp_states(n, npw, nkx0, double_pack, lsda, seed=5)
+    F = np.random.default_rng(6).standard_normal((n, 2 * npw))
+    L = oracle.lambda_fwd(psi, F)
+    if mode == 0 and lsda:
+        L = 2.0 * L  # Ortho::acceptSectionLambda doubles Lambda under lsda
+    data = oracle.backmul(L, psi, transT=True)
+    F1 = F.copy()
+    oracle.gsp_unpack(data, F1, nkx0=nkx0, double_pack=double_pack, mode=mode, lsda=lsda)
+    L1 = oracle.lambda_fwd(psi, F1)
+    assert np.abs(L1).max() < 1e-12 * np.abs(L).max()
+    # the wrong kx = 0 weight leaves the kx = 0 share of Lambda behind
+    if double_pack:
+        bad = F.copy()
+        oracle.gsp_unpack(data, bad, nkx0=0, double_pack=double_pack, mode=mode, lsda=lsda)
+        assert np.abs(oracle.lambda_fwd(psi, bad)).max() > 1e-6 * np.abs(L).max()
+
+
+@pytest.mark.parametrize("lsda", [False, True])
+def test_gsp_unpack_dynamics_projects_out_lambda(oracle, lsda):
+    """Dynamics: psi and F are sent with the kx = 0 points scaled by 1/sqrt2
+    and sqrt2; the PairCalculator returns data = -T F + Gamma^T psi with
+    Gamma = T Lambda; force = -data, then the kx = 0 points times 1/sqrt2.
+    For orthonormal states T = I and the force is again Lambda-free."""
+    n, npw, nkx0 = 9, 200, 6
+    psi = _gsp_states(n, npw, nkx0, True, lsda, seed=8)
+    F = np.random.default_rng(9).standard_normal((n, 2 * npw))
+    ps, Fs = psi.copy(), F.copy()
+    ps[:, :2 * nkx0] *= 1.0 / np.sqrt(2.0)
+    Fs[:, :2 * nkx0] *= np.sqrt(2.0)
+    S = oracle.overlap(psi, nkx0=nkx0)
+    T, _, _, st = oracle.s_to_t(S, nspin=2 if lsda else 1)
+    assert st == 0 and np.abs(T - np.eye(n)).max() < 1e-13
+    L = oracle.lambda_fwd(ps, Fs)
+    G = T @ (2.0 * L if lsda else L)
+    data = oracle.backmul(T, Fs, transT=False, alpha=-1.0)
+    oracle.backmul(G, ps, transT=True, alpha=1.0, beta=1.0, out=data)
+    force = np.zeros_like(F)
+    oracle.gsp_unpack(data, force, nkx0=nkx0, double_pack=True, mode=2, lsda=lsda)
+    assert np.abs(oracle.lambda_fwd(psi, force)).max() < 1e-12 * np.abs(L).max()
