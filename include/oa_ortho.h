@@ -219,6 +219,53 @@ int oa_ortho_step(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
                   double *S, double *T, double *psi_out, long ldout,
                   int *iters, double *err);
 
+/* ---- complex-math (k-point) path --------------------------------------
+ *
+ * The PairCalculator built with CP_PAIRCALC_USES_COMPLEX_MATH
+ * (pcFwdDeclarations.h:17-24) works on complex internal data: ZGEMM('C','N')
+ * forward (ckPairCalculator.C:840-889), Ortho on complex matrices, and
+ * ZGEMM('N','N') / ('N','C') backward (:1792-1842).  States are row-major
+ * [n][2*npw] doubles = [n][npw] complex (re, im) over the full G set of a
+ * k-point (no double packing).  n x n matrices are row-major complex [x][y],
+ * 2*n*n doubles, contiguous.  FP64 only.  Device pointers, handle stream. */
+
+/* S[x][y] = alpha * sum_k psi_x[k] conj(psi_y[k])  (ZGEMM('C','N') with
+ * right = left, ckPairCalculator.C:840-889), written exactly Hermitian. */
+int oa_zoverlap(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
+                double alpha, double *S);
+
+/* Asymmetric complex forward: L[x][y] = alpha * sum_k psi_x[k] conj(F_y[k]). */
+int oa_zlambda(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
+               const double *F, long ldF, double alpha, double *L);
+
+/* Ortho S -> T on complex matrices (ortho.C:153-971 with internalType
+ * complex; CLA_Matrix products are plain complex products): B = S/2 (nspin 1)
+ * or S, A = I; C = 3I - A B; B = B C / 2; A = C A / 2.  The stop rule is the
+ * reference's complex one: err = sqrt(Re sum (A_old - A_new)^2 / n^2) with
+ * complex squares (tolerance_check accumulates tmp * tmp, collect_error reads
+ * the real part, ortho.C:109-146, 936-971); the loop ends when err <= tol, when
+ * err is NaN (negative sum) or at maxit.  modulus != 0 uses sum |dA|^2.
+ * Synchronous; iters / err host pointers (may be NULL).  Returns
+ * OA_ERR_NOT_CONVERGED when the count reached maxit. */
+int oa_zs_to_t(oa_handle_t h, int n, const double *S, int nspin, double tol,
+               int maxit, int modulus, double *T, int *iters, double *err);
+
+/* out[y] = alpha * sum_x M[y][x] psi[x] + beta * out[y] (transM == 0,
+ * ZGEMM('N','N'), ckPairCalculator.C:1835) or conj(M[x][y]) (transM != 0,
+ * ZGEMM('N','C'), :1842). */
+int oa_zbackmul(oa_handle_t h, int n, int npw, const double *M, int transM,
+                const double *psi, long ldpsi, double alpha, double beta,
+                double *out, long ldout);
+
+/* One complex step: S = psi psi^H, S -> T (the reference's stop rule, or
+ * sum |dA|^2 when modulus != 0), out = T psi.  Under lsda (B = S) the
+ * reference rule often ends after one iteration: Re sum (dA)^2 of a Hermitian
+ * dA is negative when its off-diagonal imaginary parts dominate. */
+int oa_zortho_step(oa_handle_t h, int n, int npw, const double *psi,
+                   long ldpsi, int nspin, double tol, int maxit, int modulus,
+                   double *S, double *T, double *psi_out, long ldout,
+                   int *iters, double *err);
+
 /* ---- batched independent instances (grouped launches) -----------------
  *
  * Many independent orthonormalisation problems of uneven size -- path-integral

@@ -35,6 +35,21 @@ struct oa_context {
 };
 
 int oa_ws_require(oa_context *c, size_t bytes);
+
+// Makes the handle's device current for the scope of an API call and
+// restores the caller's device afterwards.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
 int oa_ws_require_into(oa_context *c, oa_workspace &w, size_t bytes);
 
 // ---------------------------------------------------------------- FP64 GEMM

@@ -182,6 +182,85 @@ class Handle:
                                          _ld(force), prec_code(prec)), "oa_lambda_force")
         return force
 
+    # ---- complex-math (k-point) path: states [n][2*npw] interleaved complex,
+    # n x n matrices as complex128 tensors (or [n][2n] float64)
+
+    @staticmethod
+    def _zmat(M: torch.Tensor, n: int, name: str) -> torch.Tensor:
+        if M.dtype == torch.complex128:
+            M = torch.view_as_real(M).reshape(n, 2 * n)
+        _need(M, name, n)
+        if M.shape[1] != 2 * n or not M.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous n x n complex matrix")
+        return M
+
+    def zoverlap(self, psi: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
+        """S = alpha psi psi^H as an n x n complex128 tensor."""
+        _need(psi, "psi")
+        n = psi.shape[0]
+        S = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
+        check(_lib.lib().oa_zoverlap(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), alpha,
+                                     S.data_ptr()), "oa_zoverlap")
+        return S
+
+    def zlambda(self, psi: torch.Tensor, F: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
+        """L = alpha psi F^H as an n x n complex128 tensor."""
+        _need(psi, "psi")
+        n = psi.shape[0]
+        _need(F, "F", n)
+        L = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
+        check(_lib.lib().oa_zlambda(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), _ptr(F),
+                                    _ld(F), alpha, L.data_ptr()), "oa_zlambda")
+        return L
+
+    def zs_to_t(self, S: torch.Tensor, nspin: int = 1, tol: float = 1e-15, maxit: int = 1000,
+                modulus: bool = False, raise_on_limit: bool = True):
+        """Complex Ortho S->T.  Returns (T complex128, iterations, error)."""
+        n = S.shape[0]
+        Sv = self._zmat(S, n, "S")
+        T = torch.empty((n, n), dtype=torch.complex128, device=S.device)
+        it = ctypes.c_int(0)
+        err = ctypes.c_double(0.0)
+        st = _lib.lib().oa_zs_to_t(self.h, n, _ptr(Sv), nspin, tol, maxit, 1 if modulus else 0,
+                                   T.data_ptr(), ctypes.byref(it), ctypes.byref(err))
+        self.last_iterations, self.last_error = it.value, err.value
+        if raise_on_limit or st != _lib.OA_ERR_NOT_CONVERGED:
+            check(st, "oa_zs_to_t")
+        return T, it.value, err.value
+
+    def zbackmul(self, M: torch.Tensor, psi: torch.Tensor, transM: bool = False,
+                 alpha: float = 1.0, beta: float = 0.0,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+        """out = alpha op(M) psi + beta out, op(M) = M or M^H."""
+        _need(psi, "psi")
+        n = psi.shape[0]
+        Mv = self._zmat(M, n, "M")
+        if out is None:
+            out = torch.zeros_like(psi) if beta != 0.0 else torch.empty_like(psi)
+        check(_lib.lib().oa_zbackmul(self.h, n, psi.shape[1] // 2, _ptr(Mv), 1 if transM else 0,
+                                     _ptr(psi), _ld(psi), alpha, beta, _ptr(out), _ld(out)),
+              "oa_zbackmul")
+        return out
+
+    def zortho_step(self, psi: torch.Tensor, nspin: int = 1, tol: float = 1e-15,
+                    maxit: int = 1000, modulus: bool = False):
+        """One complex step.  Returns (S, T, psi_out, iterations, error).
+        modulus: stop on sum |dA|^2 instead of the reference's Re sum dA^2."""
+        _need(psi, "psi")
+        n = psi.shape[0]
+        S = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
+        T = torch.empty_like(S)
+        out = torch.empty_like(psi)
+        it = ctypes.c_int(0)
+        err = ctypes.c_double(0.0)
+        st = _lib.lib().oa_zortho_step(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), nspin,
+                                       tol, maxit, 1 if modulus else 0, S.data_ptr(),
+                                       T.data_ptr(), _ptr(out),
+                                       _ld(out), ctypes.byref(it), ctypes.byref(err))
+        self.last_iterations, self.last_error = it.value, err.value
+        check(st, "oa_zortho_step")
+        return S, T, out, it.value, err.value
+
     def scale_kx0(self, X: torch.Tensor, nkx0: int, s: float) -> torch.Tensor:
         """X[:, :2*nkx0] *= s in place."""
         _need(X, "X")
@@ -332,14 +411,21 @@ class PairCalculator:
 
     symmetric=True  -> the psi x psi instance (S, T*psi); alpha = 2 when
                        double packing is on (gamma point), kx=0 weighted.
-    symmetric=False -> the psi x F instance (Lambda, alpha = 1)."""
+    symmetric=False -> the psi x F instance (Lambda, alpha = 1).
+    complex_math=True -> the CP_PAIRCALC_USES_COMPLEX_MATH build (k-points):
+                       complex S = psi psi^H and Lambda = psi F^H
+                       (ZGEMM('C','N')), complex T psi / Lambda^H psi; FP64
+                       only, no double packing."""
 
     def __init__(self, symmetric: bool = True, nkx0: int = 1, double_pack: bool = True,
-                 prec="fp64", handle: Handle | None = None):
+                 prec="fp64", handle: Handle | None = None, complex_math: bool = False):
         self.symmetric = symmetric
         self.nkx0 = nkx0
-        self.double_pack = double_pack
+        self.double_pack = double_pack and not complex_math
         self.prec = prec
+        self.complex_math = complex_math
+        if complex_math and prec != "fp64":
+            raise ValueError("the complex-math path runs in FP64 only")
         self.handle = handle or default_handle()
 
     def multiplyForward(self, left: torch.Tensor, right: torch.Tensor | None = None,
@@ -348,6 +434,17 @@ class PairCalculator:
         dynamics tolerance check, returned as (S, maxdev)).  Asymmetric:
         Lambda from (psi, force)."""
         h = self.handle
+        if self.complex_math:
+            if not self.symmetric:
+                if right is None:
+                    raise ValueError("asymmetric instance needs the right (force) data")
+                return h.zlambda(left, right)
+            S = h.zoverlap(left)
+            if diag is None:
+                return S
+            # Ortho::array_diag_max on the complex S (ortho.h:282-313, myabs = |z|)
+            md = (S - diag * torch.eye(S.shape[0], dtype=S.dtype, device=S.device)).abs().max()
+            return S, float(md.item())
         if self.symmetric:
             alpha = 2.0 if self.double_pack else 1.0
             md = None
@@ -363,17 +460,25 @@ class PairCalculator:
     def multiplyResult(self, M: torch.Tensor, psi: torch.Tensor, out=None,
                        alpha: float = 1.0, beta: float = 0.0):
         """Symmetric: out = alpha*T*psi + beta*out (ckPairCalculator.C:1835);
-        asymmetric: out = alpha*Lambda^T*psi + beta*out (:1842)."""
+        asymmetric: out = alpha*Lambda^T*psi + beta*out (:1842), Lambda^H
+        under complex math."""
+        if self.complex_math:
+            return self.handle.zbackmul(M, psi, transM=not self.symmetric, alpha=alpha,
+                                        beta=beta, out=out)
         return self.handle.backmul(M, psi, transM=not self.symmetric, alpha=alpha, beta=beta,
                                    out=out, prec=self.prec)
 
     def bwMultiplyDynOrthoT(self, orthoT: torch.Tensor, F: torch.Tensor, out=None,
                             beta: float = 0.0):
         """-F*T (ckPairCalculator.C:2018, alpha = -1)."""
+        if self.complex_math:
+            return self.handle.zbackmul(orthoT, F, alpha=-1.0, beta=beta, out=out)
         return self.handle.backmul(orthoT, F, transM=False, alpha=-1.0, beta=beta, out=out,
                                    prec=self.prec)
 
     def multiplyPsiV(self, orthoT: torch.Tensor, psiV: torch.Tensor, out=None):
+        if self.complex_math:
+            return self.handle.zbackmul(orthoT, psiV, out=out)
         return self.handle.backmul(orthoT, psiV, transM=False, out=out, prec=self.prec)
 
 
@@ -382,7 +487,15 @@ class Ortho:
 
     def __init__(self, nspin: int = 1, invsqr_tolerance: float = 1e-15,
                  invsqr_max_iter: int = 1000, max_tolerance: float | None = None,
-                 handle: Handle | None = None):
+                 handle: Handle | None = None, complex_math: bool = False,
+                 stop_rule: str = "reference"):
+        """complex_math: S, T, Lambda are complex128 (k-points).  stop_rule
+        (complex only): "reference" = sqrt(Re sum dA^2 / n^2) as the
+        reference's complex tolerance_check forms it, "modulus" = sum |dA|^2."""
+        if stop_rule not in ("reference", "modulus"):
+            raise ValueError(stop_rule)
+        self.complex_math = complex_math
+        self.stop_rule = stop_rule
         self.nspin = nspin
         self.lsda = nspin != 1
         self.invsqr_tolerance = invsqr_tolerance
@@ -393,6 +506,11 @@ class Ortho:
         self.error = float("nan")
 
     def start_calc(self, S: torch.Tensor) -> torch.Tensor:
+        if self.complex_math:
+            T, self.iterations, self.error = self.handle.zs_to_t(
+                S, nspin=self.nspin, tol=self.invsqr_tolerance, maxit=self.invsqr_max_iter,
+                modulus=self.stop_rule == "modulus")
+            return T
         T, self.iterations, self.error = self.handle.s_to_t(
             S, nspin=self.nspin, tol=self.invsqr_tolerance, maxit=self.invsqr_max_iter)
         return T
@@ -419,6 +537,10 @@ class Ortho:
         Lx = L * 2.0 if self.lsda else L
         G = torch.empty_like(L)
         # row-major G = T * Lx  ==  column-major G^T = Lx^T * T^T
+        if self.complex_math:
+            st = self.handle.zgemm("N", "N", n, n, n, 1.0, Lx, n, orthoT, n, 0.0, G, n)
+            check(st, "oa_zgemm")
+            return G
         st = self.handle.dgemm("N", "N", n, n, n, 1.0, Lx, n, orthoT, n, 0.0, G, n)
         check(st, "oa_dgemm")
         return G

@@ -100,6 +100,32 @@ def make_states(n: int, npw: int, eps: float = 1e-3, seed: int = 0, nkx0: int = 
     return Q.contiguous()
 
 
+def make_kpoint_states(n: int, npw: int, eps: float = 1e-3, seed: int = 0,
+                       device: str | torch.device = "cpu") -> torch.Tensor:
+    """[n][2*npw] float64 block of n complex states over the full G set of a
+    k-point (no double packing): envelope-scaled complex normal coefficients,
+    orthonormalised under <a, b> = sum_k conj(a_k) b_k by complex CholeskyQR2,
+    plus eps times an envelope-shaped complex block with rows of norm eps, so
+    that S = psi psi^H = I + O(eps).  Draws with numpy's PCG64."""
+    if n > npw:
+        raise ValueError("need n <= npw for an orthonormal block")
+    gen = np.random.default_rng(seed)
+    env = torch.from_numpy(envelope(npw)).to(device)
+
+    def block():
+        Z = torch.from_numpy(gen.standard_normal((n, 2 * npw))).to(device)
+        return torch.view_as_complex(Z.reshape(n, npw, 2)) * env[None, :]
+
+    Q = block()
+    for _ in range(2):
+        L, _info = torch.linalg.cholesky_ex(Q @ Q.conj().T)
+        Q = torch.linalg.solve_triangular(L, Q, upper=False)
+    if eps != 0.0:
+        P = block()
+        Q = Q + P * (eps / torch.linalg.vector_norm(P, dim=1, keepdim=True))
+    return torch.view_as_real(Q.contiguous()).reshape(n, 2 * npw).contiguous()
+
+
 def md_perturbation(n: int, npw: int, seed: int, nkx0: int = 1,
                     device: str | torch.device = "cuda") -> torch.Tensor:
     """An envelope-shaped complex normal block with rows of unit metric norm,

@@ -44,6 +44,16 @@
  *                        returns and the nine loop nests of op(A) x op(B) in
  *                        {N, T, C}^2 with their summation order.  Pinned
  *                        bitwise against the compiled reference zgemm_.
+ *   oao_zoverlap, oao_zlambda, oao_zs_to_t, oao_zbackmul, oao_zortho_step
+ *                        the complex-math (k-point) build
+ *                        (CP_PAIRCALC_USES_COMPLEX_MATH): ZGEMM('C','N')
+ *                        forward (ckPairCalculator.C:840-889), the Ortho loop
+ *                        on complex matrices with CLA products as
+ *                        ZGEMM('T','T') and the reference's complex stop rule
+ *                        (ortho.C:109-146, 936-971: Re of the complex sum of
+ *                        squares), ZGEMM('N','N') / ('N','C') backward
+ *                        (:1792-1842).  Every GEMM goes through a zgemm
+ *                        pointer (oao_zgemm or the compiled zgemm_).
  *   oao_gsp_unpack       the GSpace accumulation of the backward result into
  *                        the force: unpackLambda(partialResultMsg)
  *                        CP_State_GSpacePlane.C:2168-2195, unpackLambda(
@@ -557,6 +567,143 @@ void oao_gsp_unpack(int n, int npw, const double *data, int nkx0, int double_pac
       f[2 * k + 1] -= wght * d[2 * k + 1];
     }
   }
+}
+
+/* ---- complex-math (k-point) build ------------------------------------- */
+
+typedef void (*oao_zgemm_fn)(const char *ta, const char *tb, const int *m, const int *n,
+                             const int *k, const double *alpha, const double *A,
+                             const int *lda, const double *B, const int *ldb,
+                             const double *beta, double *C, const int *ldc);
+
+/* Fortran-ABI form of the ZGEMM restatement (INFO discarded, as XERBLA is
+ * silent). */
+void oao_zgemm(const char *ta, const char *tb, const int *m, const int *n, const int *k,
+               const double *alpha, const double *A, const int *lda, const double *B,
+               const int *ldb, const double *beta, double *C, const int *ldc) {
+  (void)oao_zgemm_info(*ta, *tb, *m, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);
+}
+
+static oao_zgemm_fn zpick(oao_zgemm_fn f) { return f ? f : oao_zgemm; }
+
+/* ckPairCalculator.C:840-889 with internalType complex: myGEMM promotes alpha
+ * to (alpha, 0) (:56-60) and calls ZGEMM('C','N', nY, nX, npw, alpha, right,
+ * npw, left, npw, 0, out, nY); out row-major [x][y] = alpha * sum_k
+ * left_x[k] conj(right_y[k]). */
+void oao_zpc_forward(int nX, int nY, int npw, const double *left, const double *right,
+                     double alpha, double *out, oao_zgemm_fn zg) {
+  char tC = 'C', tN = 'N';
+  double al[2] = {alpha, 0.0}, be[2] = {0.0, 0.0};
+  int m_in = nY, n_in = nX, k_in = npw;
+  zpick(zg)(&tC, &tN, &m_in, &n_in, &k_in, al, right, &k_in, left, &k_in, be, out, &m_in);
+}
+
+/* Symmetric complex forward (right = left, alpha = 1: no double pack). */
+void oao_zoverlap(int n, int npw, const double *psi, double *S, oao_zgemm_fn zg) {
+  oao_zpc_forward(n, n, npw, psi, psi, 1.0, S, zg);
+}
+
+/* Asymmetric complex forward, L = psi F^H. */
+void oao_zlambda(int n, int npw, const double *psi, const double *F, double *L,
+                 oao_zgemm_fn zg) {
+  oao_zpc_forward(n, n, npw, psi, F, 1.0, L, zg);
+}
+
+/* Plain (unconjugated) transpose of a row-major complex m x n block. */
+static void ztranspose(double *a, int m, int n) {
+  double *t = (double *)malloc((size_t)2 * m * n * sizeof(double));
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < n; j++) {
+      t[2 * ((size_t)j * m + i)] = a[2 * ((size_t)i * n + j)];
+      t[2 * ((size_t)j * m + i) + 1] = a[2 * ((size_t)i * n + j) + 1];
+    }
+  memcpy(a, t, (size_t)2 * m * n * sizeof(double));
+  free(t);
+}
+
+/* CLA_Matrix.C:528-569 with complex data: row-major dest = alpha*A*B +
+ * beta*dest through ZGEMM('T','T') and transposes, alpha and beta real. */
+static void zcla_multiply(int m, int n, int K, double alpha, const double *A, const double *B,
+                          double beta, double *dest, oao_zgemm_fn zg) {
+  char t = 'T';
+  double al[2] = {alpha, 0.0}, be[2] = {beta, 0.0};
+  if (beta != 0.0) ztranspose(dest, m, n);
+  zpick(zg)(&t, &t, &m, &n, &K, al, A, &K, B, &n, be, dest, &m);
+  ztranspose(dest, n, m);
+}
+
+/* Ortho S -> T on complex matrices: the loop of oao_s_to_t with complex
+ * arithmetic.  tolerance_check (ortho.C:950-953) accumulates tmp * tmp in
+ * complex arithmetic, the sum_double reduction adds both doubles and
+ * collect_error (:113) reads the first: err = sqrt(Re(sum) / n^2), NaN when
+ * Re(sum) < 0, which stops the loop (`error > tol` is false).  modulus != 0
+ * accumulates |tmp|^2 instead.  Returns 0, or 1 when the count reached maxit. */
+int oao_zs_to_t(int n, const double *S, int nspin, double tol, int maxit, int modulus,
+                double *T, int *iters_out, double *err_out, oao_zgemm_fn zg) {
+  const size_t nn2 = (size_t)2 * n * n;
+  double *A = T;
+  double *B = (double *)malloc(nn2 * sizeof(double));
+  double *C = (double *)malloc(nn2 * sizeof(double));
+  double *Bn = (double *)malloc(nn2 * sizeof(double));
+  double *Aold = (double *)malloc(nn2 * sizeof(double));
+  for (size_t i = 0; i < nn2; i++) B[i] = (nspin == 1) ? S[i] / 2.0 : S[i];
+  memset(A, 0, nn2 * sizeof(double));
+  for (int i = 0; i < n; i++) A[2 * ((size_t)i * n + i)] = 1;
+  int iterations = 0;
+  double error = 0.0;
+  for (;;) {
+    memset(C, 0, nn2 * sizeof(double));
+    for (int i = 0; i < n; i++) C[2 * ((size_t)i * n + i)] = 3;
+    zcla_multiply(n, n, n, -1.0, A, B, 1.0, C, zg);
+    zcla_multiply(n, n, n, 0.5, B, C, 0.0, Bn, zg);
+    memcpy(Aold, A, nn2 * sizeof(double));
+    zcla_multiply(n, n, n, 0.5, C, Aold, 0.0, A, zg);
+    double ret_re = 0.0, ret_im = 0.0;
+    for (size_t i = 0; i < nn2 / 2; i++) {
+      const double dr = Aold[2 * i] - A[2 * i], di = Aold[2 * i + 1] - A[2 * i + 1];
+      if (modulus) {
+        ret_re += dr * dr + di * di;
+      } else {
+        /* ret += tmp * tmp, complex */
+        const double sq_re = dr * dr - di * di, sq_im = dr * di + di * dr;
+        ret_re += sq_re;
+        ret_im += sq_im;
+      }
+    }
+    (void)ret_im;
+    double *sw = B; B = Bn; Bn = sw;
+    iterations++;
+    error = sqrt(ret_re / ((double)n * (double)n));
+    if (error > tol && iterations < maxit) continue;
+    break;
+  }
+  free(B); free(C); free(Bn); free(Aold);
+  if (iters_out) *iters_out = iterations;
+  if (err_out) *err_out = error;
+  return iterations >= maxit ? 1 : 0;
+}
+
+/* Complex backward (ckPairCalculator.C:1792-1842): symmetric ZGEMM('N','N',
+ * npw, n, n, alpha, psi, npw, M, n, beta, out, npw), out_y = alpha * sum_x
+ * M[y][x] psi_x + beta out_y; asymmetric ZGEMM('N','C'), out_y = alpha * sum_x
+ * conj(M[x][y]) psi_x + beta out_y.  M row-major complex n x n. */
+void oao_zbackmul(int n, int npw, const double *M, int transM, const double *psi, double alpha,
+                  double beta, double *out, oao_zgemm_fn zg) {
+  char tN = 'N', tC = 'C';
+  double al[2] = {alpha, 0.0}, be[2] = {beta, 0.0};
+  zpick(zg)(&tN, transM ? &tC : &tN, &npw, &n, &n, al, psi, &npw, M, &n, be, out, &npw);
+}
+
+/* One complex step: S = psi psi^H, S -> T (reference stop rule unless
+ * modulus), out = T psi. */
+int oao_zortho_step(int n, int npw, const double *psi, int nspin, double tol, int maxit,
+                    int modulus, double *S, double *T, double *psi_out, int *iters,
+                    double *err, oao_zgemm_fn zg) {
+  oao_zoverlap(n, npw, psi, S, zg);
+  int st = oao_zs_to_t(n, S, nspin, tol, maxit, modulus, T, iters, err, zg);
+  memset(psi_out, 0, (size_t)2 * n * npw * sizeof(double));
+  oao_zbackmul(n, npw, T, 0, psi, 1.0, 0.0, psi_out, zg);
+  return st;
 }
 
 /* max |S(psi) - occ*I| with S in the 2Re - G0 metric of oao_overlap. */

@@ -84,6 +84,16 @@ def lib() -> ctypes.CDLL:
                                      ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
                                      ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.oao_zgemm_info.restype = ctypes.c_int
+        _vp = ctypes.c_void_p
+        _i, _d = ctypes.c_int, ctypes.c_double
+        L.oao_zoverlap.argtypes = [_i, _i, _dp, _dp, _vp]
+        L.oao_zlambda.argtypes = [_i, _i, _dp, _dp, _dp, _vp]
+        L.oao_zs_to_t.restype = _i
+        L.oao_zs_to_t.argtypes = [_i, _dp, _i, _d, _i, _i, _dp, _ip, _dp, _vp]
+        L.oao_zbackmul.argtypes = [_i, _i, _dp, _i, _dp, _d, _d, _dp, _vp]
+        L.oao_zortho_step.restype = _i
+        L.oao_zortho_step.argtypes = [_i, _i, _dp, _i, _d, _i, _i, _dp, _dp, _dp, _ip, _dp,
+                                      _vp]
         L.oao_gsp_unpack.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
@@ -223,6 +233,82 @@ def ortho_step(psi: np.ndarray, nkx0: int = 1, nspin: int = 1, tol: float = 1e-1
     st = lib().oao_ortho_step(n, K // 2, _p(psi), nkx0, nspin, tol, maxit, _p(S),
                               _p(T), _p(out), ctypes.byref(it), ctypes.byref(err),
                               nplanes, gemm_ptr(gemm))
+    return dict(S=S, T=T, psi_out=out, iters=it.value, err=err.value, status=st)
+
+
+def zgemm_ptr(which: str):
+    """'oracle' -> oao_zgemm, 'ref' -> the reference's compiled zgemm_."""
+    if which == "oracle":
+        return ctypes.cast(lib().oao_zgemm, ctypes.c_void_p)
+    if which == "ref":
+        return ctypes.cast(ref().zgemm_, ctypes.c_void_p)
+    raise ValueError(which)
+
+
+def _zsq(M: np.ndarray) -> np.ndarray:
+    M = np.ascontiguousarray(M, dtype=np.complex128)
+    return M
+
+
+def _pc(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def zoverlap(psi: np.ndarray, gemm="oracle") -> np.ndarray:
+    """Complex-math forward S = psi psi^H (n x n complex128); psi is the
+    interleaved [n][2*npw] float64 block."""
+    n, K = psi.shape
+    psi = np.ascontiguousarray(psi)
+    S = np.empty((n, n), dtype=np.complex128)
+    lib().oao_zoverlap(n, K // 2, _p(psi), _pc(S), zgemm_ptr(gemm))
+    return S
+
+
+def zlambda(psi: np.ndarray, F: np.ndarray, gemm="oracle") -> np.ndarray:
+    n, K = psi.shape
+    psi, F = np.ascontiguousarray(psi), np.ascontiguousarray(F)
+    L = np.empty((n, n), dtype=np.complex128)
+    lib().oao_zlambda(n, K // 2, _p(psi), _p(F), _pc(L), zgemm_ptr(gemm))
+    return L
+
+
+def zs_to_t(S: np.ndarray, nspin: int = 1, tol: float = 1e-15, maxit: int = 1000,
+            modulus: bool = False, gemm="oracle"):
+    """Complex Ortho S->T.  Returns (T, iterations, error, status)."""
+    S = _zsq(S)
+    n = S.shape[0]
+    T = np.empty_like(S)
+    it = ctypes.c_int(0)
+    err = ctypes.c_double(0)
+    st = lib().oao_zs_to_t(n, _pc(S), nspin, tol, maxit, 1 if modulus else 0, _pc(T),
+                           ctypes.byref(it), ctypes.byref(err), zgemm_ptr(gemm))
+    return T, it.value, err.value, st
+
+
+def zbackmul(M: np.ndarray, psi: np.ndarray, transM: bool = False, alpha: float = 1.0,
+             beta: float = 0.0, out: np.ndarray | None = None, gemm="oracle") -> np.ndarray:
+    n, K = psi.shape
+    M = _zsq(M)
+    psi = np.ascontiguousarray(psi)
+    if out is None:
+        out = np.zeros((n, K))
+    lib().oao_zbackmul(n, K // 2, _pc(M), 1 if transM else 0, _p(psi), alpha, beta, _p(out),
+                       zgemm_ptr(gemm))
+    return out
+
+
+def zortho_step(psi: np.ndarray, nspin: int = 1, tol: float = 1e-15, maxit: int = 1000,
+                modulus: bool = False, gemm="oracle"):
+    n, K = psi.shape
+    psi = np.ascontiguousarray(psi)
+    S = np.empty((n, n), dtype=np.complex128)
+    T = np.empty_like(S)
+    out = np.empty((n, K))
+    it = ctypes.c_int(0)
+    err = ctypes.c_double(0)
+    st = lib().oao_zortho_step(n, K // 2, _p(psi), nspin, tol, maxit, 1 if modulus else 0,
+                               _pc(S), _pc(T), _p(out),
+                               ctypes.byref(it), ctypes.byref(err), zgemm_ptr(gemm))
     return dict(S=S, T=T, psi_out=out, iters=it.value, err=err.value, status=st)
 
 

@@ -1,0 +1,156 @@
+"""GPU parity of the complex-math (k-point) path -- oa_zoverlap, oa_zlambda,
+oa_zs_to_t, oa_zbackmul, oa_zortho_step and the complex PairCalculator /
+Ortho mirrors -- against the CPU oracle (itself pinned by numpy's complex
+products and Hermitian eigensolver, the real oracle and the reference's
+k-point state files).
+
+Tolerances: the FP64-MFMA products accumulate in another order than the
+reference's ZGEMM loops, so results agree to 1e-12 relative to the largest
+entry; Hermitian symmetry of S is exact (upper triangle mirrored).
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from openatom_amd.ortho import Ortho, PairCalculator  # noqa: E402
+from openatom_amd.workload import make_kpoint_states, make_states  # noqa: E402
+
+DEV = "cuda"
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
+
+
+def cplx(psi):
+    return psi[:, 0::2] + 1j * psi[:, 1::2]
+
+
+@pytest.mark.parametrize("n,npw", [(64, 2048), (37, 1000), (200, 5000), (1, 64)])
+def test_zoverlap_and_zlambda(handle, oracle, n, npw):
+    psi_h = make_kpoint_states(n, npw, eps=1e-2, seed=n).numpy()
+    F_h = make_kpoint_states(n, npw, eps=1e-1, seed=n + 1).numpy()
+    S = handle.zoverlap(dev(psi_h))
+    L = handle.zlambda(dev(psi_h), dev(F_h), alpha=-0.5)
+    torch.cuda.synchronize()
+    S_h, L_h = S.cpu().numpy(), L.cpu().numpy()
+    assert rel(S_h, oracle.zoverlap(psi_h)) < 1e-13
+    assert np.array_equal(S_h, S_h.conj().T)
+    assert rel(L_h, -0.5 * oracle.zlambda(psi_h, F_h)) < 1e-13
+
+
+@pytest.mark.parametrize("kpt", [0, 1])
+def test_zoverlap_gwbse_fixture(handle, kpt):
+    d = np.load(os.path.join(GOLDEN, f"gwbse_kpt{kpt}.npz"))
+    psi = np.ascontiguousarray(d["psi"]).view(np.float64)
+    S = handle.zoverlap(dev(psi)).cpu().numpy()
+    assert np.abs(S - np.eye(S.shape[0])).max() < 5e-15
+
+
+@pytest.mark.parametrize("nspin,modulus", [(1, False), (1, True), (2, True)])
+def test_zs_to_t_vs_oracle(handle, oracle, nspin, modulus):
+    n = 96
+    psi_h = make_kpoint_states(n, 3000, eps=2e-2, seed=5).numpy()
+    S_h = oracle.zoverlap(psi_h)
+    if nspin == 1:
+        S_h = 2.0 * S_h
+    Tw, itw, errw, stw = oracle.zs_to_t(S_h, nspin=nspin, tol=1e-14, maxit=60, modulus=modulus)
+    T, it, err = handle.zs_to_t(dev(S_h), nspin=nspin, tol=1e-14, maxit=60, modulus=modulus)
+    assert stw == 0 and it == itw
+    assert rel(T.cpu().numpy(), Tw) < 1e-12
+    if modulus:  # the reference rule may stop early on a NaN error
+        B = S_h / 2.0 if nspin == 1 else S_h
+        Th = T.cpu().numpy()
+        assert np.abs(Th @ B @ Th - np.eye(n)).max() < 1e-12
+
+
+def test_zs_to_t_reference_rule_under_lsda(handle, oracle):
+    """The reference's complex stop rule ends after one iteration with a NaN
+    error on this block, on the device as in the oracle."""
+    psi_h = make_kpoint_states(30, 400, eps=1e-2, seed=8).numpy()
+    S_h = oracle.zoverlap(psi_h)
+    Tw, itw, errw, _ = oracle.zs_to_t(S_h, nspin=2, tol=1e-15, maxit=50)
+    T, it, err = handle.zs_to_t(dev(S_h), nspin=2, tol=1e-15, maxit=50)
+    assert itw == it == 1 and np.isnan(err) and np.isnan(errw)
+    assert rel(T.cpu().numpy(), Tw) < 1e-13
+
+
+def test_zs_to_t_real_input_matches_real_path(handle):
+    psi = make_states(64, 2000, eps=1e-2, seed=9, device=DEV, rng="torch")
+    S = handle.overlap_sym(psi, nkx0=1, prec="fp64")
+    Tr, itr, _ = handle.s_to_t(S, nspin=2, tol=1e-14, maxit=50)
+    Tz, itz, _ = handle.zs_to_t(S.to(torch.complex128), nspin=2, tol=1e-14, maxit=50)
+    torch.cuda.synchronize()
+    assert itr == itz
+    assert rel(Tz.real.cpu().numpy(), Tr.cpu().numpy()) < 1e-13
+    assert not Tz.imag.any().item()
+
+
+@pytest.mark.parametrize("transM", [False, True])
+@pytest.mark.parametrize("n,npw", [(80, 3000), (33, 513)])
+def test_zbackmul_vs_oracle(handle, oracle, transM, n, npw):
+    rng = np.random.default_rng(n)
+    psi_h = make_kpoint_states(n, npw, eps=0.0, seed=n).numpy()
+    M_h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    out0 = rng.standard_normal(psi_h.shape)
+    want = oracle.zbackmul(M_h, psi_h, transM=transM, alpha=-0.5, beta=2.0, out=out0.copy())
+    got = handle.zbackmul(dev(M_h), dev(psi_h), transM=transM, alpha=-0.5, beta=2.0,
+                          out=dev(out0))
+    torch.cuda.synchronize()
+    assert rel(got.cpu().numpy(), want) < 1e-13
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_zortho_step_vs_oracle(handle, oracle, nspin):
+    n, npw = 256, 4096
+    psi_h = make_kpoint_states(n, npw, eps=1e-2, seed=10).numpy()
+    want = oracle.zortho_step(psi_h, nspin=nspin, tol=1e-14, maxit=50, modulus=True)
+    S, T, out, it, err = handle.zortho_step(dev(psi_h), nspin=nspin, tol=1e-14, maxit=50,
+                                            modulus=True)
+    torch.cuda.synchronize()
+    assert it == want["iters"]
+    assert rel(S.cpu().numpy(), want["S"]) < 1e-13
+    assert rel(T.cpu().numpy(), want["T"]) < 1e-12
+    assert rel(out.cpu().numpy(), want["psi_out"]) < 1e-12
+    c = cplx(out.cpu().numpy())
+    occ = 2.0 if nspin == 1 else 1.0
+    assert np.abs(c @ c.conj().T - occ * np.eye(n)).max() < 1e-12
+
+
+def test_complex_pair_calculator_round_trip(handle, oracle):
+    """The complex mirrors chained as the reference's instance set runs them:
+    S, Ortho's T, T psi; Lambda, Gamma = T Lambda, Lambda^H psi."""
+    n, npw = 48, 1500
+    psi_h = make_kpoint_states(n, npw, eps=5e-3, seed=11).numpy()
+    F_h = make_kpoint_states(n, npw, eps=1e-1, seed=12).numpy()
+    psi, F = dev(psi_h), dev(F_h)
+    sym = PairCalculator(symmetric=True, complex_math=True, handle=handle)
+    asym = PairCalculator(symmetric=False, complex_math=True, handle=handle)
+    ortho = Ortho(nspin=1, complex_math=True, stop_rule="modulus", invsqr_tolerance=1e-14,
+                  invsqr_max_iter=50, handle=handle)
+    S, md = sym.multiplyForward(psi, diag=1.0)
+    T = ortho.start_calc(2.0 * S)
+    out = sym.multiplyResult(T, psi)
+    L = asym.multiplyForward(psi, F)
+    G = ortho.gamma(T, L)
+    back = asym.multiplyResult(L, psi)
+    torch.cuda.synchronize()
+    S_h = oracle.zoverlap(psi_h)
+    assert abs(md - np.abs(S_h - np.eye(n)).max()) < 1e-14
+    Tw, _, _, _ = oracle.zs_to_t(2.0 * S_h, nspin=1, tol=1e-14, maxit=50, modulus=True)
+    assert rel(T.cpu().numpy(), Tw) < 1e-12
+    assert rel(out.cpu().numpy(), oracle.zbackmul(Tw, psi_h)) < 1e-12
+    L_h = oracle.zlambda(psi_h, F_h)
+    assert rel(L.cpu().numpy(), L_h) < 1e-13
+    assert rel(G.cpu().numpy(), Tw @ L_h) < 1e-12
+    assert rel(back.cpu().numpy(), oracle.zbackmul(L_h, psi_h, transM=True)) < 1e-12

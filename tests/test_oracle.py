@@ -18,7 +18,7 @@ import os
 import numpy as np
 import pytest
 
-from openatom_amd.workload import make_states, metric_weights
+from openatom_amd.workload import make_kpoint_states, make_states, metric_weights
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -401,3 +401,115 @@ def test_gsp_unpack_dynamics_projects_out_lambda(oracle, lsda):
     force = np.zeros_like(F)
     oracle.gsp_unpack(data, force, nkx0=nkx0, double_pack=True, mode=2, lsda=lsda)
     assert np.abs(oracle.lambda_fwd(psi, force)).max() < 1e-12 * np.abs(L).max()
+
+
+# ---- complex-math (k-point) build ----------------------------------------
+
+def _cplx(psi):
+    return psi[:, 0::2] + 1j * psi[:, 1::2]
+
+
+def test_zoverlap_is_psi_psih(oracle):
+    psi = make_kpoint_states(20, 333, eps=1e-2, seed=2).numpy()
+    S = oracle.zoverlap(psi)
+    c = _cplx(psi)
+    want = c @ c.conj().T
+    assert np.abs(S - want).max() < 1e-14 * np.abs(want).max()
+    assert np.array_equal(S, S.conj().T)  # ZGEMM('C','N') terms are exact conjugates
+    L = oracle.zlambda(psi, psi[::-1].copy())
+    want = c @ c[::-1].conj().T
+    assert np.abs(L - want).max() < 1e-14 * np.abs(want).max()
+    if oracle.ref_available():
+        assert np.array_equal(oracle.zoverlap(psi, gemm="ref"), S)
+
+
+@pytest.mark.parametrize("kpt", [0, 1])
+def test_zoverlap_gwbse_fixture(oracle, kpt):
+    """The reference's k-point states: the forward of the complex-math build
+    gives the identity."""
+    d = np.load(os.path.join(GOLDEN, f"gwbse_kpt{kpt}.npz"))
+    psi = np.ascontiguousarray(d["psi"]).view(np.float64)
+    S = oracle.zoverlap(psi)
+    assert np.abs(S - np.eye(S.shape[0])).max() < 5e-15
+    res = oracle.zortho_step(psi, nspin=2, tol=1e-15, maxit=20)
+    assert res["status"] == 0
+    assert np.abs(res["T"] - np.eye(S.shape[0])).max() < 1e-14
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_zs_to_t_closed_form(oracle, nspin):
+    """T = B^(-1/2) of the Hermitian B = S/2 (nspin 1) or S, from numpy's
+    Hermitian eigensolver."""
+    psi = make_kpoint_states(40, 500, eps=2e-2, seed=3).numpy()
+    S = oracle.zoverlap(psi)
+    if nspin == 1:
+        S = 2.0 * S
+    T, it, err, st = oracle.zs_to_t(S, nspin=nspin, tol=1e-14, maxit=100, modulus=True)
+    assert st == 0 and err <= 1e-14
+    B = S / 2.0 if nspin == 1 else S
+    lam, V = np.linalg.eigh(B)
+    Tc = (V * lam ** -0.5) @ V.conj().T
+    assert np.abs(T - Tc).max() < 1e-13
+    assert np.abs(T @ B @ T - np.eye(40)).max() < 1e-13
+    assert np.abs(T - T.conj().T).max() < 1e-14
+
+
+def test_zs_to_t_real_input_equals_real_oracle(oracle):
+    """On a real symmetric S the complex loop performs the real one's
+    arithmetic exactly (imaginary parts stay zero, the complex products add
+    exact zeros), so it must reproduce the real oracle bit for bit -- which is
+    itself pinned to the compiled reference."""
+    psi = make_states(32, 400, eps=1e-2, seed=4).numpy()
+    S = oracle.overlap(psi, 1)
+    Tr, itr, errr, str_ = oracle.s_to_t(S, nspin=2, tol=1e-15, maxit=50)
+    Tz, itz, errz, stz = oracle.zs_to_t(S.astype(np.complex128), nspin=2, tol=1e-15, maxit=50)
+    assert (itr, str_) == (itz, stz) and errr == errz
+    assert np.array_equal(Tz.real, Tr) and not Tz.imag.any()
+    if oracle.ref_available():
+        Tf, itf, errf, stf = oracle.zs_to_t(S.astype(np.complex128), nspin=2, tol=1e-15,
+                                            maxit=50, gemm="ref")
+        assert np.array_equal(Tf, Tz) and itf == itz
+
+
+def test_zs_to_t_reference_stop_rule(oracle):
+    """The reference's complex rule takes the real part of sum (dA)^2: it ends
+    the loop either below the tolerance or on a NaN error (negative sum);
+    the modulus rule always runs to the tolerance."""
+    psi = make_kpoint_states(24, 300, eps=5e-2, seed=5).numpy()
+    S = oracle.zoverlap(psi)
+    Tm, itm, errm, stm = oracle.zs_to_t(S, nspin=2, tol=1e-14, maxit=60, modulus=True)
+    Tr, itr, errr, str_ = oracle.zs_to_t(S, nspin=2, tol=1e-14, maxit=60)
+    assert stm == 0 and errm <= 1e-14
+    assert str_ == 0 and (errr <= 1e-14 or np.isnan(errr)) and itr <= itm
+
+
+@pytest.mark.parametrize("transM", [False, True])
+def test_zbackmul_is_the_complex_product(oracle, transM):
+    rng = np.random.default_rng(7)
+    psi = make_kpoint_states(16, 200, eps=0.0, seed=6).numpy()
+    M = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+    out0 = rng.standard_normal(psi.shape)
+    got = oracle.zbackmul(M, psi, transM=transM, alpha=-0.5, beta=2.0, out=out0.copy())
+    op = M.conj().T if transM else M
+    want = -0.5 * (op @ _cplx(psi)) + 2.0 * _cplx(out0)
+    assert np.abs(_cplx(got) - want).max() < 1e-13 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_zortho_step_orthonormalises(oracle, nspin):
+    psi = make_kpoint_states(30, 400, eps=1e-2, seed=8).numpy()
+    res = oracle.zortho_step(psi, nspin=nspin, tol=1e-15, maxit=50, modulus=True)
+    assert res["status"] == 0
+    c = _cplx(res["psi_out"])
+    occ = 2.0 if nspin == 1 else 1.0
+    assert np.abs(c @ c.conj().T - occ * np.eye(30)).max() < 1e-13
+
+
+def test_zortho_step_reference_rule_under_lsda(oracle):
+    """Under lsda B = S, and the first change dA = (I - S)/2 is dominated by
+    off-diagonal complex entries, whose squares have negative real parts:
+    the reference's complex rule reads a NaN error and stops after one
+    iteration, leaving the block far from orthonormal."""
+    psi = make_kpoint_states(30, 400, eps=1e-2, seed=8).numpy()
+    res = oracle.zortho_step(psi, nspin=2, tol=1e-15, maxit=50)
+    assert res["status"] == 0 and res["iters"] == 1 and np.isnan(res["err"])
