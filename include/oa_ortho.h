@@ -41,6 +41,7 @@ extern "C" {
 #define OA_ERR_NO_DEVICE (-3)     /* no usable gfx950 device                 */
 #define OA_ERR_UNSUPPORTED (-4)   /* precision/variant not available         */
 #define OA_ERR_CAPACITY (-5)      /* request exceeds a documented limit      */
+#define OA_ERR_IO (-6)            /* a state file cannot be opened or parsed */
 
 /* Arithmetic of the O(n^2 npw) products (overlap, Lambda, back-multiply).
  * S->T, the G=0 term and all reductions stay FP64 in every mode.
@@ -265,6 +266,49 @@ int oa_zortho_step(oa_handle_t h, int n, int npw, const double *psi,
                    long ldpsi, int nspin, double tol, int maxit, int modulus,
                    double *S, double *T, double *psi_out, long ldout,
                    int *iters, double *err);
+
+/* ---- state files and the kx = 0 plane layout -----------------------------
+ *
+ * A state file (utility/util.C:1107-1330 readState, :1955-2180
+ * writeStateFile) is a header `ncoef nx ny nz` and one `re im kx ky kz`
+ * record per plane wave.  format: 0 text, 1 binary, 2 gzip text, 3 gzip
+ * binary (2 and 3 open `<path>.gz`, as the reference does).  Host calls,
+ * no device work; they return OA_ERR_IO when a file cannot be opened,
+ * parsed or written. */
+
+/* Read one state.  double_pack != 0 (gamma point): the file lists the upper
+ * half of the kx = 0 plane, the kx > 0 points and G = 0 last; reading stops
+ * at G = 0 and the lower half of the kx = 0 plane is rebuilt by conjugate
+ * symmetry (para_grp_parse.C:178-264 flip_data_set), giving the memory
+ * layout [lower half | G = 0 | upper half | kx > 0]; the flip's consistency
+ * checks return OA_ERR_IO.  double_pack == 0 (k-points): the records as
+ * listed.  *npts receives the number of points of the layout, *nkx0 the
+ * points with kx = 0 (the leading block of weight 1 in the path metric; 0
+ * without double packing), grid[3] nx ny nz.  coef holds 2*capacity doubles,
+ * kx/ky/kz capacity ints (may be NULL); when npts exceeds capacity the call
+ * returns OA_ERR_CAPACITY after setting *npts, *nkx0 and grid. */
+int oa_state_read(const char *path, int format, int double_pack, int capacity,
+                  double *coef, int *kx, int *ky, int *kz, int *npts,
+                  int *nkx0, int *grid);
+
+/* Write one state as writeStateFile does: points ordered by (kx, ky, kz)
+ * (sort_psi_output, util.C:2180-2215); with double_pack the points after
+ * G = 0 in that order and then G = 0 (the lower half of the kx = 0 plane is
+ * dropped), header count = points written.  Text records use the
+ * reference's "%g %g %d %d %d \n". */
+int oa_state_write(const char *path, int format, int double_pack, int npts,
+                   const double *coef, const int *kx, const int *ky,
+                   const int *kz, const int *grid);
+
+/* Device form of flip_data_set for a block of n states: rows of nfile
+ * points in file order (upper half of the kx = 0 plane: nhalf points,
+ * kx > 0 points, G = 0 last) -> rows of nfile + nhalf points in memory order,
+ * and back (contract drops the lower half).  Row-major, (re, im) pairs,
+ * leading dimensions in doubles. */
+int oa_kx0_expand(oa_handle_t h, int n, int nfile, int nhalf,
+                  const double *in, long ldin, double *out, long ldout);
+int oa_kx0_contract(oa_handle_t h, int n, int nfile, int nhalf,
+                    const double *in, long ldin, double *out, long ldout);
 
 /* ---- batched independent instances (grouped launches) -----------------
  *

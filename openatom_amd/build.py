@@ -24,7 +24,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
          "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["oa_f64.hip", "oa_split.hip", "oa_batch.hip", "oa_complex.hip", "oa_kpoint.hip", "oa_api.hip"]
+SOURCES = ["oa_f64.hip", "oa_split.hip", "oa_batch.hip", "oa_complex.hip", "oa_kpoint.hip", "oa_statefile.hip",
+           "oa_api.hip"]
 
 
 def _compile(src: str) -> str:
@@ -45,7 +46,7 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-lz", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

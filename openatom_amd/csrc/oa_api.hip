@@ -117,6 +117,7 @@ const char *oa_status_string(int st) {
     case OA_ERR_NO_DEVICE: return "no usable gfx950 device";
     case OA_ERR_UNSUPPORTED: return "unsupported precision or variant";
     case OA_ERR_CAPACITY: return "request exceeds a limit (workspace under capture, size)";
+    case OA_ERR_IO: return "state file cannot be opened, parsed or written";
     default: return st > 0 ? "invalid argument (BLAS INFO index)" : "unknown status";
   }
 }

@@ -7,13 +7,16 @@ Spin.0_Kpt.{0,1}_Bead.0_Temper.0/state{1..8}.out.gz; format: a header line
 sphere with G = 0 first).  Expected output: the Hermitian overlap
 S = Psi^H Psi by the reference's ZGEMM compiled from src_mathlib/zgemm.f
 (oracle/_ref/librefblas.so): ZGEMM('C', 'N', 8, 8, ncoef, 1, A, ncoef, A,
-ncoef, 0, S, 8) with A(:, j) = state j.  Run from the repository root:
+ncoef, 0, S, 8) with A(:, j) = state j.  Also copies state1 and state2 of
+k-point 0 unchanged (gwbse_kpt0_state{1,2}.out.gz) for the state-file
+reader's tests.  Run from the repository root:
 python tests/golden/make_gwbse.py
 """
 from __future__ import annotations
 
 import gzip
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -58,6 +61,11 @@ def main() -> None:
                             grid=grid, S_cm=S)
         print(f"kpt {kpt}: {nst} states x {ncoef} plane waves, "
               f"max|S - I| = {np.abs(S.T - np.eye(nst)).max():.2e}")
+    # two of the state files themselves, unchanged, for the state-file reader
+    d = os.path.join(SRC, "Spin.0_Kpt.0_Bead.0_Temper.0")
+    for j in (1, 2):
+        shutil.copyfile(os.path.join(d, f"state{j}.out.gz"),
+                        os.path.join(OUT, f"gwbse_kpt0_state{j}.out.gz"))
 
 
 if __name__ == "__main__":
