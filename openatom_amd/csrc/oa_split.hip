@@ -716,7 +716,13 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
     }
   }
 
-  // epilogue: combine levels in FP64
+  // epilogue: combine levels in FP64.  MODE 1 works on groups of 4 rows per
+  // lane: it loads T's per-row exponent and FP64 diagonal together with the
+  // psi~ digits of those rows, and tests beta and the tile edge outside the
+  // element loop, so an interior tile pays one memory round trip per group
+  // (a per-element bounds test kept the compiler from hoisting the loads: one
+  // round trip per output).
+  const bool edge = (m0 + TILE > a.M) || (n0 + TILE > a.N);
 #pragma unroll
   for (int i = 0; i < WI; i++)
 #pragma unroll
@@ -724,29 +730,7 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
     const int ln = yrow + j * 32 + (lane & 31);
     const int nn = n0 + ln;
     const int xr = xrow + i * 32;
-    // MODE 1: psi~[m][nn] for this lane's 16 rows, rebuilt from all S digits
-    double pd[16];
-    int ez = 0;
-    if (MODE == 1) {
-      ez = chunk_exp(__longlong_as_double((long long)a.colmax[min(nn, a.N - 1) / a.kchunk]));
-#pragma unroll
-      for (int v = 0; v < 16; v++) pd[v] = 0.0;
-#pragma unroll
-      for (int p = S - 1; p >= 0; p--) {
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-          double q4[4];
-          const size_t off = (size_t)p * a.dbs +
-                             kb_off<DT>(min(nn, a.N - 1), m0 + xr + 8 * g + 4 * (lane >> 5),
-                                        (size_t)a.Kp);
-          load_digit_quad<DT>(a.DB, off, q4);
-#pragma unroll
-          for (int r = 0; r < 4; r++) pd[4 * g + r] += q4[r] * rinv(p);
-        }
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
+    auto level_sum = [&](int v) {
       double val = 0.0;
       if (DT == DT_I8) {
 #pragma unroll
@@ -754,25 +738,73 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       } else {
         val = facc[i][j][v];
       }
-      const int lm = xr + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-      if (MODE == 0) {
-        const int e = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
-        a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
-            ldexp(val, 2 * (e - 7));
-      } else if (MODE == 2) {
-        const int ex = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
-        const int ey = chunk_exp(__longlong_as_double((long long)a.colmax[z]));
-        a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
-            ldexp(val, ex + ey - 14);
-      } else {
-        const int m = m0 + lm;
-        if (m < a.M && nn < a.N) {
-          const double al = (nn < a.cw) ? a.alpha_w : a.alpha;
-          const double r = al * (a.diagv[m] * ldexp(pd[v], ez - 7) +
-                                 ldexp(val, a.fexp[m] + ez - 14));
-          double *o = a.out + (size_t)m * a.ldo + nn;
-          *o = (a.beta != 0.0) ? r + a.beta * (*o) : r;
+      return val;
+    };
+    if (MODE != 1) {
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const double val = level_sum(v);
+        const int lm = xr + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (MODE == 0) {
+          const int e = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
+          a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
+              ldexp(val, 2 * (e - 7));
+        } else {
+          const int ex = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
+          const int ey = chunk_exp(__longlong_as_double((long long)a.colmax[z]));
+          a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
+              ldexp(val, ex + ey - 14);
         }
+      }
+      continue;
+    }
+    const int ez =
+        chunk_exp(__longlong_as_double((long long)a.colmax[min(nn, a.N - 1) / a.kchunk]));
+    const double al = (nn < a.cw) ? a.alpha_w : a.alpha;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      // elements v = 4g + r sit in rows mg + r, mg = m0 + xr + 8g + 4 (lane / 32)
+      const int mg = m0 + xr + 8 * g + 4 * (lane >> 5);
+      double pd[4] = {0.0, 0.0, 0.0, 0.0}, dv[4];
+      int fe[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = min(mg + r, a.M - 1);
+        dv[r] = a.diagv[m];
+        fe[r] = a.fexp[m];
+      }
+      // psi~[m][nn] for these rows, rebuilt from all S digits
+#pragma unroll
+      for (int p = S - 1; p >= 0; p--) {
+        double q4[4];
+        load_digit_quad<DT>(a.DB, (size_t)p * a.dbs + kb_off<DT>(min(nn, a.N - 1), mg,
+                                                                   (size_t)a.Kp), q4);
+#pragma unroll
+        for (int r = 0; r < 4; r++) pd[r] += q4[r] * rinv(p);
+      }
+      auto value = [&](int r) {
+        return al * (dv[r] * ldexp(pd[r], ez - 7) +
+                     ldexp(level_sum(4 * g + r), fe[r] + ez - 14));
+      };
+      double *o = a.out + (size_t)mg * a.ldo + nn;
+      if (!edge) {
+        if (a.beta == 0.0) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) o[(size_t)r * a.ldo] = value(r);
+        } else {
+          double old[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) old[r] = o[(size_t)r * a.ldo];
+#pragma unroll
+          for (int r = 0; r < 4; r++) o[(size_t)r * a.ldo] = value(r) + a.beta * old[r];
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (mg + r < a.M && nn < a.N) {
+            double *ov = o + (size_t)r * a.ldo;
+            *ov = (a.beta != 0.0) ? value(r) + a.beta * (*ov) : value(r);
+          }
       }
     }
   }
