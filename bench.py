@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batched64", "serving"],
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batched64", "serving", "lambda",
+                                                               "kpoint"],
                     default="large")
     ap.add_argument("--rates", default="50,200,500,1000,2000",
                     help="serving: offered request rates (req/s) to sweep")
@@ -311,6 +312,238 @@ def run_serving(args):
     print(json.dumps(line), flush=True)
 
 
+def _timed(fn, stream, reps):
+    """Mean ms of `fn` over reps launches, HIP events on `stream`."""
+    import torch
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / reps
+
+
+def run_lambda(args):
+    """Lambda round trip of the large instance (n=1024, npw=262144; psi and
+    the force F of the same shape): Lambda = psi F^T (asymmetric forward), then
+    the GSpace force update F -= w .* (Psi Lambda), w = 1/2 on the kx = 0
+    points, fused into the backward product (oa_lambda_force)."""
+    import torch
+
+    from openatom_amd.ortho import Handle
+    from openatom_amd.workload import make_force, make_states
+    n, npw = 1024, 262144
+    torch.cuda.set_device(0)
+    psi = make_states(n, npw, eps=1e-3, seed=0, device="cuda", rng="torch")
+    F0 = make_force(n, npw, seed=1, device="cuda", rng="torch")
+    torch.cuda.synchronize()
+    h = Handle(0)
+    stream = torch.cuda.current_stream()
+    h.use_stream(stream)
+    res, ref = {}, None
+    for prec in [m for m in args.modes.split(",") if m]:
+        F = F0.clone()
+        L = torch.empty((n, n), dtype=torch.float64, device="cuda")
+
+        def fwd():
+            h.lambda_fwd(psi, F0, alpha=1.0, prec=prec, L=L)
+
+        def back():
+            h.lambda_force(L, psi, F, transM=True, nkx0=1, wkx0=0.5, alpha=-1.0, beta=1.0,
+                           prec=prec)
+        for _ in range(args.warmup):
+            fwd()
+            back()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fwd()
+            back()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        f_ms = _timed(fwd, stream, 3)
+        b_ms = _timed(back, stream, 3)
+        # one clean round trip for the error against FP64
+        F.copy_(F0)
+        fwd()
+        back()
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = (L.clone(), F.clone())
+        err_l = float((L - ref[0]).abs().max() / ref[0].abs().max())
+        err_f = float((F - ref[1]).abs().max() / ref[1].abs().max())
+        res[prec] = {"round_trips_per_s": 1.0 / wall, "ms": 1e3 * wall, "forward_ms": f_ms,
+                     "force_update_ms": b_ms, "lambda_rel_err_vs_first_mode": err_l,
+                     "force_rel_err_vs_first_mode": err_f}
+        h.synchronize()
+    head = res.get("fp64") or next(iter(res.values()))
+    hp = "fp64" if "fp64" in res else next(iter(res))
+    if hp == "fp64":
+        ops = 2.0 * n * n * (2 * npw)
+        peak, unit, kern = PEAK_FP64_TFLOPS, "TFLOP/s", "gemm_f64_kernel (force update, Lambda^T psi)"
+    else:
+        s_, lean = _slices(hp)
+        nl = s_ - 2 if lean else s_ - 1
+        ops = nl * (nl + 1) // 2 * 2.0 * n * n * (2 * npw)
+        peak, unit, kern = PEAK_INT8_TOPS, "TOP/s", "digit_nt_kernel back-multiply (force update)"
+    achieved = ops / (head["force_update_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": "Lambda round trips/s (psi F^T forward + GSpace force update), n=1024, "
+                  "npw=262144",
+        "value": head["round_trips_per_s"], "unit": "round trips/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if hp == "fp64" else hp,
+        "data": "synthetic psi (as the large instance) and F i.i.d. normal std 0.1, seeds 0/1",
+        "config": {"workload": "Lambda round trip, n=1024, npw=262144, nkx0=1", "nstates": n,
+                   "npw": npw, "headline_mode": hp},
+        "modes": res,
+        "roofline": {"kernel": kern, "bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": unit, "frac": achieved / peak, "traffic": None,
+                     "algorithmic_per_launch": f"{ops:.4g} ops"},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_lambda(n, npw, min(16, os.cpu_count() or 1))
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_lambda(n, npw, threads):
+    """The Lambda round trip on the host with the reference's compiled DGEMM:
+    forward psi F^T and backward Lambda^T psi over a sample of plane waves,
+    K split over `threads` planes run in parallel (one PairCalculator tile per
+    core), scaled to npw."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    from oracle import oracle as o
+    o.build()
+    gemm = "ref" if o.ref_available() else "oracle"
+    o.set_threads(1)
+    npw_s = min(npw, 4096)
+    rng = np.random.default_rng(0)
+    psi = rng.standard_normal((n, 2 * npw_s)) * 1e-2
+    F = rng.standard_normal((n, 2 * npw_s)) * 1e-1
+    edges = [2 * (npw_s * p // threads) for p in range(threads + 1)]
+    parts = [(np.ascontiguousarray(psi[:, a:b]), np.ascontiguousarray(F[:, a:b]))
+             for a, b in zip(edges[:-1], edges[1:])]
+    with cf.ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        L = sum(ex.map(lambda pf: o.lambda_fwd(pf[0], pf[1], gemm=gemm), parts))
+        t1 = time.perf_counter()
+        list(ex.map(lambda pf: o.backmul(L, pf[0], transT=True, gemm=gemm), parts))
+        t2 = time.perf_counter()
+    t = (t2 - t0) * (npw / npw_s)
+    return {"value": 1.0 / t, "unit": "round trips/s", "cores": threads,
+            "kind": "reference" if gemm == "ref" else "port",
+            "sample": (f"n={n}: forward psi F^T and backward Lambda^T psi over npw={npw_s} of "
+                       f"{npw} plane waves ({threads} planes in parallel) scaled by "
+                       f"{npw / npw_s:g}; measured {t1 - t0:.2f}s fwd, {t2 - t1:.2f}s bwd")}
+
+
+def run_kpoint(args):
+    """Complex-math (k-point) step: S = psi psi^H, complex S->T (modulus stop
+    rule), T psi, on a block of the large instance's size in memory: n=1024
+    states of npw=262144 complex coefficients (full G set)."""
+    import torch
+
+    from openatom_amd.ortho import Handle
+    from openatom_amd.workload import make_kpoint_states
+    n, npw = 1024, 262144
+    torch.cuda.set_device(0)
+    psi = make_kpoint_states(n, npw, eps=1e-3, seed=0, device="cuda", rng="torch")
+    torch.cuda.synchronize()
+    h = Handle(0)
+    stream = torch.cuda.current_stream()
+    h.use_stream(stream)
+    S = torch.empty((n, n), dtype=torch.complex128, device="cuda")
+    T = torch.empty_like(S)
+    outb = torch.empty_like(psi)
+
+    def ovl():
+        S.copy_(h.zoverlap(psi))
+
+    def s2t():
+        T.copy_(h.zs_to_t(S, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True)[0])
+
+    def back():
+        h.zbackmul(T, psi, out=outb)
+
+    def step():  # one oa_zortho_step call: J psi formed once for both products
+        return h.zortho_step(psi, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    out = res[2]
+    # phase split from the separate calls (each forms J psi itself)
+    ovl()
+    s2t()
+    o_ms, b_ms = _timed(ovl, stream, 3), _timed(back, stream, 3)
+    _, iters, err = h.zs_to_t(S, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True)
+    S2 = h.zoverlap(out)
+    resid = float((S2 - torch.eye(n, dtype=S2.dtype, device="cuda")).abs().max())
+    ops_bm = 8.0 * n * n * npw  # complex T psi: n^2 npw complex MACs
+    achieved = ops_bm / (b_ms * 1e-3) / 1e12
+    line = {
+        "metric": "complex k-point ortho steps/s (S=psi psi^H + complex S->T + T psi), FP64",
+        "value": 1.0 / wall, "unit": "ortho steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic complex states (envelope-scaled complex normal, complex CholeskyQR2, "
+                "eps=1e-3 perturbation; torch generator seed 0)",
+        "config": {"workload": "complex k-point step, n=1024, npw=262144 complex (full G set)",
+                   "nstates": n, "npw": npw, "nspin": 2, "s_to_t_tol": args.tol,
+                   "s_to_t_maxit": args.maxit, "stop_rule": "modulus"},
+        "overlap_ms": o_ms, "backmul_ms": b_ms, "iterations": iters, "s_to_t_error": err,
+        "timed_call": "oa_zortho_step (phase times from separate oa_zoverlap / oa_zbackmul calls)",
+        "residual": resid,
+        "roofline": {"kernel": "complex T psi: J psi pass + two gemm_f64_kernel launches "
+                               "(Re(T) psi + Im(T) J psi)",
+                     "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "algorithmic_per_launch": f"8*n^2*npw = {ops_bm:.4g} FP64 flop (two launches)"},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_kpoint(n, npw, iters)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_kpoint(n, npw, iters):
+    """The complex step on one host core with the reference's compiled ZGEMM:
+    forward and backward over a sample of npw_s coefficients (scaled to npw),
+    one complex S->T iteration at n_s = 256 (scaled by (n / n_s)^3)."""
+    from oracle import oracle as o
+    from openatom_amd.workload import make_kpoint_states
+    o.build()
+    gemm = "ref" if o.ref_available() else "oracle"
+    npw_s, n_s = 1024, 256
+    psi = make_kpoint_states(n, npw_s, eps=1e-3, seed=0).numpy()
+    t0 = time.perf_counter()
+    S = o.zoverlap(psi, gemm=gemm)
+    t1 = time.perf_counter()
+    # a dense matrix: the netlib ZGEMM skips the zero entries of its B operand
+    o.zbackmul(S, psi, gemm=gemm)
+    t2 = time.perf_counter()
+    Ss = S[:n_s, :n_s]
+    t3 = time.perf_counter()
+    o.zs_to_t(Ss, nspin=2, tol=0.0, maxit=1, modulus=True, gemm=gemm)
+    t4 = time.perf_counter()
+    t = (t2 - t0) * (npw / npw_s) + iters * (t4 - t3) * (n / n_s) ** 3
+    return {"value": 1.0 / t, "unit": "ortho steps/s", "cores": 1,
+            "kind": "reference" if gemm == "ref" else "port",
+            "sample": (f"n={n}: complex forward and backward over npw={npw_s} of {npw} "
+                       f"coefficients scaled by {npw / npw_s:g}, plus {iters} complex S->T "
+                       f"iterations timed at n={n_s} and scaled by {(n / n_s) ** 3:g}; measured "
+                       f"{t1 - t0:.2f}s fwd, {t2 - t1:.2f}s bwd, {t4 - t3:.2f}s/iter")}
+
+
 def cpu_baseline(n, npw, iters, threads):
     """Reference call sequence on the host with the reference's compiled
     netlib DGEMM, on a bounded sample: forward and backward over a slice of
@@ -361,6 +594,12 @@ def main():
         return
     if args.workload == "serving":
         run_serving(args)
+        return
+    if args.workload == "lambda":
+        run_lambda(args)
+        return
+    if args.workload == "kpoint":
+        run_kpoint(args)
         return
     import torch
 

@@ -101,25 +101,37 @@ def make_states(n: int, npw: int, eps: float = 1e-3, seed: int = 0, nkx0: int = 
 
 
 def make_kpoint_states(n: int, npw: int, eps: float = 1e-3, seed: int = 0,
-                       device: str | torch.device = "cpu") -> torch.Tensor:
+                       device: str | torch.device = "cpu", rng: str = "numpy") -> torch.Tensor:
     """[n][2*npw] float64 block of n complex states over the full G set of a
     k-point (no double packing): envelope-scaled complex normal coefficients,
     orthonormalised under <a, b> = sum_k conj(a_k) b_k by complex CholeskyQR2,
     plus eps times an envelope-shaped complex block with rows of norm eps, so
-    that S = psi psi^H = I + O(eps).  Draws with numpy's PCG64."""
+    that S = psi psi^H = I + O(eps).  rng="numpy" draws with numpy's PCG64,
+    rng="torch" with torch's generator on `device` (the large blocks)."""
     if n > npw:
         raise ValueError("need n <= npw for an orthonormal block")
-    gen = np.random.default_rng(seed)
-    env = torch.from_numpy(envelope(npw)).to(device)
+    if rng == "numpy":
+        gen = np.random.default_rng(seed)
+    else:
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
+    env = _on_device(("env1", npw), lambda: envelope(npw), device)
 
     def block():
-        Z = torch.from_numpy(gen.standard_normal((n, 2 * npw))).to(device)
+        if rng == "numpy":
+            Z = torch.from_numpy(gen.standard_normal((n, 2 * npw))).to(device)
+        else:
+            Z = torch.randn((n, 2 * npw), generator=gen, dtype=torch.float64, device=device)
         return torch.view_as_complex(Z.reshape(n, npw, 2)) * env[None, :]
 
     Q = block()
     for _ in range(2):
+        # invert the n x n factor and apply it with a GEMM: a triangular solve
+        # with npw right-hand sides needs a library workspace that large
+        # blocks exceed
         L, _info = torch.linalg.cholesky_ex(Q @ Q.conj().T)
-        Q = torch.linalg.solve_triangular(L, Q, upper=False)
+        eye = torch.eye(n, dtype=L.dtype, device=L.device)
+        Q = torch.linalg.solve_triangular(L, eye, upper=False) @ Q
     if eps != 0.0:
         P = block()
         Q = Q + P * (eps / torch.linalg.vector_norm(P, dim=1, keepdim=True))
