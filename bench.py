@@ -446,7 +446,7 @@ def cpu_baseline_lambda(n, npw, threads):
 def run_kpoint(args):
     """Complex-math (k-point) step: S = psi psi^H, complex S->T (modulus stop
     rule), T psi, on a block of the large instance's size in memory: n=1024
-    states of npw=262144 complex coefficients (full G set)."""
+    states of npw=262144 complex coefficients (full G set); modes as --modes."""
     import torch
 
     from openatom_amd.ortho import Handle
@@ -461,57 +461,76 @@ def run_kpoint(args):
     S = torch.empty((n, n), dtype=torch.complex128, device="cuda")
     T = torch.empty_like(S)
     outb = torch.empty_like(psi)
+    eye = torch.eye(n, dtype=torch.complex128, device="cuda")
+    res = {}
+    for prec in [m for m in args.modes.split(",") if m]:
+        def ovl():
+            h.zoverlap(psi, prec=prec, S=S)
 
-    def ovl():
-        S.copy_(h.zoverlap(psi))
+        def back():
+            h.zbackmul(T, psi, out=outb, prec=prec)
 
-    def s2t():
+        def step():  # one oa_zortho_step call: J psi formed once for both products
+            return h.zortho_step(psi, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True,
+                                 prec=prec)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = step()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        out, iters, err = r[2], r[3], r[4]
+        # phase split from the separate calls (each forms J psi itself)
+        ovl()
         T.copy_(h.zs_to_t(S, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True)[0])
-
-    def back():
-        h.zbackmul(T, psi, out=outb)
-
-    def step():  # one oa_zortho_step call: J psi formed once for both products
-        return h.zortho_step(psi, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = step()
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
-    out = res[2]
-    # phase split from the separate calls (each forms J psi itself)
-    ovl()
-    s2t()
-    o_ms, b_ms = _timed(ovl, stream, 3), _timed(back, stream, 3)
-    _, iters, err = h.zs_to_t(S, nspin=2, tol=args.tol, maxit=args.maxit, modulus=True)
-    S2 = h.zoverlap(out)
-    resid = float((S2 - torch.eye(n, dtype=S2.dtype, device="cuda")).abs().max())
-    ops_bm = 8.0 * n * n * npw  # complex T psi: n^2 npw complex MACs
-    achieved = ops_bm / (b_ms * 1e-3) / 1e12
+        o_ms, b_ms = _timed(ovl, stream, 3), _timed(back, stream, 3)
+        resid = float((h.zoverlap(out) - eye).abs().max())
+        res[prec] = {"steps_per_s": 1.0 / wall, "ms_per_step": 1e3 * wall, "overlap_ms": o_ms,
+                     "backmul_ms": b_ms, "iterations": iters, "s_to_t_error": err,
+                     "residual": resid}
+        del out, r
+        h.synchronize()
+    ok = {k: v for k, v in res.items() if v["residual"] <= RESID_TARGET}
+    hp = max(ok or res, key=lambda k: res[k]["steps_per_s"])
+    head = res[hp]
+    if hp == "fp64":
+        ops = 8.0 * n * n * npw  # complex T psi: n^2 npw complex MACs
+        peak, unit = PEAK_FP64_TFLOPS, "TFLOP/s"
+        kern = "complex T psi: J psi pass + two gemm_f64_kernel launches (Re(T) psi + Im(T) J psi)"
+        alg = f"8*n^2*npw = {ops:.4g} FP64 flop"
+    else:
+        s_, lean = _slices(hp)
+        nl = s_ - 2 if lean else s_ - 1
+        prods = nl * (nl + 1) // 2
+        ops = 2 * prods * 2.0 * n * n * (2 * npw)
+        peak, unit = PEAK_INT8_TOPS, "TOP/s"
+        kern = f"complex T psi: two digit back-multiplies ({hp}: {prods} int8 products each)"
+        alg = f"2 x {prods} products x 2*n*n*(2*npw) = {ops:.4g} int8 op"
+    achieved = ops / (head["backmul_ms"] * 1e-3) / 1e12
+    fp = res.get("fp64")
     line = {
-        "metric": "complex k-point ortho steps/s (S=psi psi^H + complex S->T + T psi), FP64",
-        "value": 1.0 / wall, "unit": "ortho steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * wall, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": "complex k-point ortho steps/s (S=psi psi^H + complex S->T + T psi) split-prec "
+                  "vs FP64 at resid<=1e-10",
+        "value": head["steps_per_s"], "unit": "ortho steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if hp == "fp64" else hp,
         "data": "synthetic complex states (envelope-scaled complex normal, complex CholeskyQR2, "
                 "eps=1e-3 perturbation; torch generator seed 0)",
         "config": {"workload": "complex k-point step, n=1024, npw=262144 complex (full G set)",
                    "nstates": n, "npw": npw, "nspin": 2, "s_to_t_tol": args.tol,
-                   "s_to_t_maxit": args.maxit, "stop_rule": "modulus"},
-        "overlap_ms": o_ms, "backmul_ms": b_ms, "iterations": iters, "s_to_t_error": err,
+                   "s_to_t_maxit": args.maxit, "stop_rule": "modulus", "headline_mode": hp},
         "timed_call": "oa_zortho_step (phase times from separate oa_zoverlap / oa_zbackmul calls)",
-        "residual": resid,
-        "roofline": {"kernel": "complex T psi: J psi pass + two gemm_f64_kernel launches "
-                               "(Re(T) psi + Im(T) J psi)",
-                     "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "algorithmic_per_launch": f"8*n^2*npw = {ops_bm:.4g} FP64 flop (two launches)"},
+        "residual": head["residual"], "iterations": head["iterations"], "modes": res,
+        "fp64_steps_per_s": fp["steps_per_s"] if fp else None,
+        "speedup_vs_fp64": head["steps_per_s"] / fp["steps_per_s"] if fp else None,
+        "roofline": {"kernel": kern, "bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": unit, "frac": achieved / peak, "traffic": None,
+                     "algorithmic_per_launch": alg},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline_kpoint(n, npw, iters)
+        line["cpu_baseline"] = cpu_baseline_kpoint(n, npw, head["iterations"])
     print(json.dumps(line), flush=True)
 
 

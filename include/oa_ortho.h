@@ -228,16 +228,20 @@ int oa_ortho_step(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
  * ZGEMM('N','N') / ('N','C') backward (:1792-1842).  States are row-major
  * [n][2*npw] doubles = [n][npw] complex (re, im) over the full G set of a
  * k-point (no double packing).  n x n matrices are row-major complex [x][y],
- * 2*n*n doubles, contiguous.  FP64 only.  Device pointers, handle stream. */
+ * 2*n*n doubles, contiguous.  Device pointers, handle stream.  prec selects
+ * the products' arithmetic as for the real calls (OA_PREC_FP64 or a digit
+ * split: Re S on the symmetric digit overlap, Im S and Lambda on the
+ * asymmetric digit product of psi with i psi, T psi as two digit
+ * back-multiplies); S -> T is FP64 in every mode. */
 
 /* S[x][y] = alpha * sum_k psi_x[k] conj(psi_y[k])  (ZGEMM('C','N') with
  * right = left, ckPairCalculator.C:840-889), written exactly Hermitian. */
 int oa_zoverlap(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
-                double alpha, double *S);
+                double alpha, double *S, int prec);
 
 /* Asymmetric complex forward: L[x][y] = alpha * sum_k psi_x[k] conj(F_y[k]). */
 int oa_zlambda(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
-               const double *F, long ldF, double alpha, double *L);
+               const double *F, long ldF, double alpha, double *L, int prec);
 
 /* Ortho S -> T on complex matrices (ortho.C:153-971 with internalType
  * complex; CLA_Matrix products are plain complex products): B = S/2 (nspin 1)
@@ -256,7 +260,7 @@ int oa_zs_to_t(oa_handle_t h, int n, const double *S, int nspin, double tol,
  * ZGEMM('N','C'), :1842). */
 int oa_zbackmul(oa_handle_t h, int n, int npw, const double *M, int transM,
                 const double *psi, long ldpsi, double alpha, double beta,
-                double *out, long ldout);
+                double *out, long ldout, int prec);
 
 /* One complex step: S = psi psi^H, S -> T (the reference's stop rule, or
  * sum |dA|^2 when modulus != 0), out = T psi.  Under lsda (B = S) the
@@ -265,7 +269,7 @@ int oa_zbackmul(oa_handle_t h, int n, int npw, const double *M, int transM,
 int oa_zortho_step(oa_handle_t h, int n, int npw, const double *psi,
                    long ldpsi, int nspin, double tol, int maxit, int modulus,
                    double *S, double *T, double *psi_out, long ldout,
-                   int *iters, double *err);
+                   int *iters, double *err, int prec);
 
 /* ---- state files and the kx = 0 plane layout -----------------------------
  *

@@ -110,18 +110,18 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_backmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
     _sig(L.oa_lambda_force, _i, _vp, _i, _i, _vp, _i, _vp, _l, _i, _d, _d, _d, _vp, _l, _i)
     _sig(L.oa_scale_kx0, _i, _vp, _i, _i, _vp, _l, _i, _d)
-    _sig(L.oa_zoverlap, _i, _vp, _i, _i, _vp, _l, _d, _vp)
+    _sig(L.oa_zoverlap, _i, _vp, _i, _i, _vp, _l, _d, _vp, _i)
     _sig(L.oa_state_read, _i, ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp,
          ctypes.POINTER(_i), ctypes.POINTER(_i), _vp)
     _sig(L.oa_state_write, _i, ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp, _vp)
     _sig(L.oa_kx0_expand, _i, _vp, _i, _i, _i, _vp, _l, _vp, _l)
     _sig(L.oa_kx0_contract, _i, _vp, _i, _i, _i, _vp, _l, _vp, _l)
-    _sig(L.oa_zlambda, _i, _vp, _i, _i, _vp, _l, _vp, _l, _d, _vp)
+    _sig(L.oa_zlambda, _i, _vp, _i, _i, _vp, _l, _vp, _l, _d, _vp, _i)
     _sig(L.oa_zs_to_t, _i, _vp, _i, _vp, _i, _d, _i, _i, _vp, ctypes.POINTER(_i),
          ctypes.POINTER(_d))
-    _sig(L.oa_zbackmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l)
+    _sig(L.oa_zbackmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
     _sig(L.oa_zortho_step, _i, _vp, _i, _i, _vp, _l, _i, _d, _i, _i, _vp, _vp, _vp, _l,
-         ctypes.POINTER(_i), ctypes.POINTER(_d))
+         ctypes.POINTER(_i), ctypes.POINTER(_d), _i)
     _sig(L.oa_ortho_step, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp, _l,
          ctypes.POINTER(_i), ctypes.POINTER(_d))
     _dp = ctypes.POINTER(ctypes.c_double)

@@ -16,6 +16,9 @@
 //       M psi = Re(M) psi + Im(M) (J psi);
 //   * n x n matrices are kept as separate real and imaginary planes, and a
 //     complex product is four real products in two launches of two problems.
+// With a split precision the forward and backward products run on the digit
+// kernels instead (Re S on the symmetric overlap, Im S and Lambda on the
+// asymmetric product, T psi as two back-multiplies); S -> T stays FP64.
 #include <math.h>
 
 #include <vector>
@@ -153,20 +156,28 @@ size_t off_planes(int n, int K) { return al256((size_t)n * K * sizeof(double)); 
 size_t plane_bytes(int n) { return al256((size_t)n * n * sizeof(double)); }
 size_t off_kws(int n, int K) { return off_planes(n, K) + 2 * plane_bytes(n); }
 
-size_t zoverlap_ws_bytes(int n, int K) {
-  const size_t kws = oa_overlap_f64_ws_bytes(n, K) > oa_ntsplit_f64_ws_bytes(n, n, K)
-                         ? oa_overlap_f64_ws_bytes(n, K)
-                         : oa_ntsplit_f64_ws_bytes(n, n, K);
+bool is_split(int prec) { return (prec & 0xff) != OA_PREC_FP64; }
+
+size_t zoverlap_ws_bytes(int n, int K, int prec) {
+  size_t kws = oa_overlap_f64_ws_bytes(n, K);
+  if (oa_ntsplit_f64_ws_bytes(n, n, K) > kws) kws = oa_ntsplit_f64_ws_bytes(n, n, K);
+  if (is_split(prec)) {
+    if (oa_split_ws_bytes(prec, n, K) > kws) kws = oa_split_ws_bytes(prec, n, K);
+    if (oa_split_lambda_ws_bytes(prec, n, K) > kws) kws = oa_split_lambda_ws_bytes(prec, n, K);
+  }
   return off_kws(n, K) + kws;
 }
 
-size_t zbackmul_ws_bytes(int n, int K) { return off_kws(n, K); }
+size_t zbackmul_ws_bytes(int n, int K, int prec) {
+  return off_kws(n, K) + (is_split(prec) ? oa_split_ws_bytes(prec, n, K) : 0);
+}
 
 // Hermitian (sym) or general (!sym) product L[x][y] = alpha sum_k psi_x[k]
 // conj(phi_y[k]); `J` receives J phi ([n][K], leading dimension K), the
 // real and imaginary planes and the kernel workspace follow it in ws.
 int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld,
-             const double *phi, long ldphi, double alpha, double *L, char *ws, size_t ws_bytes) {
+             const double *phi, long ldphi, double alpha, double *L, char *ws, size_t ws_bytes,
+             int prec) {
   const int K = 2 * npw;
   double *J = reinterpret_cast<double *>(ws);
   double *Lr = reinterpret_cast<double *>(ws + off_planes(n, K));
@@ -175,12 +186,22 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
   const size_t kbytes = ws_bytes - off_kws(n, K);
   OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, phi, ldphi, J, (long)K);
   int st;
-  if (sym)
-    st = oa_launch_overlap_f64(s, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws, kbytes);
-  else
-    st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, phi, ldphi, alpha, Lr, n, kws, kbytes);
-  if (st) return st;
-  st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, J, K, alpha, Li, n, kws, kbytes);
+  if (is_split(prec)) {
+    if (sym)
+      st = oa_launch_overlap_split(s, prec, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws,
+                                   kbytes);
+    else
+      st = oa_launch_lambda_split(s, prec, n, K, psi, ld, phi, ldphi, alpha, Lr, kws, kbytes);
+    if (st) return st;
+    st = oa_launch_lambda_split(s, prec, n, K, psi, ld, J, K, alpha, Li, kws, kbytes);
+  } else {
+    if (sym)
+      st = oa_launch_overlap_f64(s, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws, kbytes);
+    else
+      st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, phi, ldphi, alpha, Lr, n, kws, kbytes);
+    if (st) return st;
+    st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, J, K, alpha, Li, n, kws, kbytes);
+  }
   if (st) return st;
   if (sym)
     OA_KLAUNCH(zherm_kernel, (long)n * n, s, n, Lr, Li, L);
@@ -192,7 +213,8 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
 // out = alpha op(M) psi + beta out, op(M) = M or M^H; `J` holds J psi
 // already when have_j.
 int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const double *psi,
-              long ld, double alpha, double beta, double *out, long ldo, char *ws, bool have_j) {
+              long ld, double alpha, double beta, double *out, long ldo, char *ws,
+              size_t ws_bytes, bool have_j, int prec) {
   const int K = 2 * npw;
   double *J = reinterpret_cast<double *>(ws);
   double *Mr = reinterpret_cast<double *>(ws + off_planes(n, K));
@@ -200,6 +222,16 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
   if (!have_j) OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, psi, ld, J, (long)K);
   OA_KLAUNCH(zplanes_kernel, (long)n * n, s, (long)n * n, M, 1.0, Mr, Mi);
   // Re(M^H) = Mr^T and Im(M^H) = -Mi^T
+  if (is_split(prec)) {
+    char *kws = ws + off_kws(n, K);
+    const size_t kbytes = ws_bytes - off_kws(n, K);
+    const int p = prec & ~OA_PREC_REUSE_DIGITS;
+    int st = oa_launch_backmul_split(s, p, n, K, Mr, transM, psi, ld, 0, alpha, beta, out, ldo,
+                                     kws, kbytes);
+    if (st) return st;
+    return oa_launch_backmul_split(s, p, n, K, Mi, transM, J, K, 0, transM ? -alpha : alpha, 1.0,
+                                   out, ldo, kws, kbytes);
+  }
   oa_gemm_problem p{n, K, n, Mr, n, psi, ld, out, ldo, alpha, beta, 0.0};
   int st = oa_launch_gemm_f64(s, transM != 0, false, &p, 1, nullptr);
   if (st) return st;
@@ -272,7 +304,7 @@ size_t zs_to_t_ws_bytes(int n) {
 extern "C" {
 
 int oa_zoverlap(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, double alpha,
-                double *S) {
+                double *S, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
   if (n < 0) return 2;
   if (npw < 0) return 3;
@@ -285,14 +317,14 @@ int oa_zoverlap(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, do
     OA_HIP_CHECK(hipMemsetAsync(S, 0, 2 * (size_t)n * n * sizeof(double), c->stream));
     return OA_SUCCESS;
   }
-  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw));
+  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw, prec));
   if (st) return st;
   return zforward(c->stream, true, n, npw, psi, ldpsi, psi, ldpsi, alpha, S, (char *)c->ws.ptr,
-                  c->ws.bytes);
+                  c->ws.bytes, prec);
 }
 
 int oa_zlambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, const double *F,
-               long ldF, double alpha, double *L) {
+               long ldF, double alpha, double *L, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
   if (n < 0) return 2;
   if (npw < 0) return 3;
@@ -307,10 +339,10 @@ int oa_zlambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, con
     OA_HIP_CHECK(hipMemsetAsync(L, 0, 2 * (size_t)n * n * sizeof(double), c->stream));
     return OA_SUCCESS;
   }
-  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw));
+  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw, prec));
   if (st) return st;
   return zforward(c->stream, false, n, npw, psi, ldpsi, F, ldF, alpha, L, (char *)c->ws.ptr,
-                  c->ws.bytes);
+                  c->ws.bytes, prec);
 }
 
 int oa_zs_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int maxit,
@@ -329,7 +361,7 @@ int oa_zs_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int
 }
 
 int oa_zbackmul(oa_handle_t c, int n, int npw, const double *M, int transM, const double *psi,
-                long ldpsi, double alpha, double beta, double *out, long ldout) {
+                long ldpsi, double alpha, double beta, double *out, long ldout, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
   if (n < 0) return 2;
   if (npw < 0) return 3;
@@ -340,15 +372,15 @@ int oa_zbackmul(oa_handle_t c, int n, int npw, const double *M, int transM, cons
   if (ldout < 2L * npw) return 11;
   if (n == 0 || npw == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
-  const int st = oa_ws_require(c, zbackmul_ws_bytes(n, 2 * npw));
+  const int st = oa_ws_require(c, zbackmul_ws_bytes(n, 2 * npw, prec));
   if (st) return st;
   return zbackward(c->stream, n, npw, M, transM, psi, ldpsi, alpha, beta, out, ldout,
-                   (char *)c->ws.ptr, false);
+                   (char *)c->ws.ptr, c->ws.bytes, false, prec);
 }
 
 int oa_zortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, int nspin,
                    double tol, int maxit, int modulus, double *S, double *T, double *psi_out,
-                   long ldout, int *iters, double *err) {
+                   long ldout, int *iters, double *err, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
   if (n < 0) return 2;
   if (npw < 1) return 3;
@@ -361,16 +393,19 @@ int oa_zortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi,
   if (n == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
   const int K = 2 * npw;
-  int st = oa_ws_require(c, zoverlap_ws_bytes(n, K));
+  size_t need = zoverlap_ws_bytes(n, K, prec);
+  if (zbackmul_ws_bytes(n, K, prec) > need) need = zbackmul_ws_bytes(n, K, prec);
+  int st = oa_ws_require(c, need);
   if (!st) st = oa_ws_require_into(c, c->ns, zs_to_t_ws_bytes(n));
   if (st) return st;
   char *ws = (char *)c->ws.ptr;
-  st = zforward(c->stream, true, n, npw, psi, ldpsi, psi, ldpsi, 1.0, S, ws, c->ws.bytes);
+  st = zforward(c->stream, true, n, npw, psi, ldpsi, psi, ldpsi, 1.0, S, ws, c->ws.bytes, prec);
   if (st) return st;
   const int st_ns = zs_to_t_run(c, n, S, nspin, tol, maxit, modulus, T, iters, err);
   if (st_ns != OA_SUCCESS && st_ns != OA_ERR_NOT_CONVERGED) return st_ns;
   // the forward left J psi at the start of the workspace
-  st = zbackward(c->stream, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, ws, true);
+  st = zbackward(c->stream, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, ws, c->ws.bytes,
+                 true, prec);
   return st ? st : st_ns;
 }
 

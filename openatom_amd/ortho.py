@@ -194,23 +194,26 @@ class Handle:
             raise ValueError(f"{name} must be a contiguous n x n complex matrix")
         return M
 
-    def zoverlap(self, psi: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
+    def zoverlap(self, psi: torch.Tensor, alpha: float = 1.0, prec="fp64",
+                 S: torch.Tensor | None = None) -> torch.Tensor:
         """S = alpha psi psi^H as an n x n complex128 tensor."""
         _need(psi, "psi")
         n = psi.shape[0]
-        S = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
+        if S is None:
+            S = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
         check(_lib.lib().oa_zoverlap(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), alpha,
-                                     S.data_ptr()), "oa_zoverlap")
+                                     S.data_ptr(), prec_code(prec)), "oa_zoverlap")
         return S
 
-    def zlambda(self, psi: torch.Tensor, F: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
+    def zlambda(self, psi: torch.Tensor, F: torch.Tensor, alpha: float = 1.0,
+                prec="fp64") -> torch.Tensor:
         """L = alpha psi F^H as an n x n complex128 tensor."""
         _need(psi, "psi")
         n = psi.shape[0]
         _need(F, "F", n)
         L = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
         check(_lib.lib().oa_zlambda(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), _ptr(F),
-                                    _ld(F), alpha, L.data_ptr()), "oa_zlambda")
+                                    _ld(F), alpha, L.data_ptr(), prec_code(prec)), "oa_zlambda")
         return L
 
     def zs_to_t(self, S: torch.Tensor, nspin: int = 1, tol: float = 1e-15, maxit: int = 1000,
@@ -230,7 +233,7 @@ class Handle:
 
     def zbackmul(self, M: torch.Tensor, psi: torch.Tensor, transM: bool = False,
                  alpha: float = 1.0, beta: float = 0.0,
-                 out: torch.Tensor | None = None) -> torch.Tensor:
+                 out: torch.Tensor | None = None, prec="fp64") -> torch.Tensor:
         """out = alpha op(M) psi + beta out, op(M) = M or M^H."""
         _need(psi, "psi")
         n = psi.shape[0]
@@ -238,12 +241,12 @@ class Handle:
         if out is None:
             out = torch.zeros_like(psi) if beta != 0.0 else torch.empty_like(psi)
         check(_lib.lib().oa_zbackmul(self.h, n, psi.shape[1] // 2, _ptr(Mv), 1 if transM else 0,
-                                     _ptr(psi), _ld(psi), alpha, beta, _ptr(out), _ld(out)),
-              "oa_zbackmul")
+                                     _ptr(psi), _ld(psi), alpha, beta, _ptr(out), _ld(out),
+                                     prec_code(prec)), "oa_zbackmul")
         return out
 
     def zortho_step(self, psi: torch.Tensor, nspin: int = 1, tol: float = 1e-15,
-                    maxit: int = 1000, modulus: bool = False):
+                    maxit: int = 1000, modulus: bool = False, prec="fp64"):
         """One complex step.  Returns (S, T, psi_out, iterations, error).
         modulus: stop on sum |dA|^2 instead of the reference's Re sum dA^2."""
         _need(psi, "psi")
@@ -256,7 +259,8 @@ class Handle:
         st = _lib.lib().oa_zortho_step(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), nspin,
                                        tol, maxit, 1 if modulus else 0, S.data_ptr(),
                                        T.data_ptr(), _ptr(out),
-                                       _ld(out), ctypes.byref(it), ctypes.byref(err))
+                                       _ld(out), ctypes.byref(it), ctypes.byref(err),
+                                       prec_code(prec))
         self.last_iterations, self.last_error = it.value, err.value
         check(st, "oa_zortho_step")
         return S, T, out, it.value, err.value
@@ -414,8 +418,8 @@ class PairCalculator:
     symmetric=False -> the psi x F instance (Lambda, alpha = 1).
     complex_math=True -> the CP_PAIRCALC_USES_COMPLEX_MATH build (k-points):
                        complex S = psi psi^H and Lambda = psi F^H
-                       (ZGEMM('C','N')), complex T psi / Lambda^H psi; FP64
-                       only, no double packing."""
+                       (ZGEMM('C','N')), complex T psi / Lambda^H psi; no
+                       double packing."""
 
     def __init__(self, symmetric: bool = True, nkx0: int = 1, double_pack: bool = True,
                  prec="fp64", handle: Handle | None = None, complex_math: bool = False):
@@ -424,8 +428,6 @@ class PairCalculator:
         self.double_pack = double_pack and not complex_math
         self.prec = prec
         self.complex_math = complex_math
-        if complex_math and prec != "fp64":
-            raise ValueError("the complex-math path runs in FP64 only")
         self.handle = handle or default_handle()
 
     def multiplyForward(self, left: torch.Tensor, right: torch.Tensor | None = None,
@@ -438,8 +440,8 @@ class PairCalculator:
             if not self.symmetric:
                 if right is None:
                     raise ValueError("asymmetric instance needs the right (force) data")
-                return h.zlambda(left, right)
-            S = h.zoverlap(left)
+                return h.zlambda(left, right, prec=self.prec)
+            S = h.zoverlap(left, prec=self.prec)
             if diag is None:
                 return S
             # Ortho::array_diag_max on the complex S (ortho.h:282-313, myabs = |z|)
@@ -464,7 +466,7 @@ class PairCalculator:
         under complex math."""
         if self.complex_math:
             return self.handle.zbackmul(M, psi, transM=not self.symmetric, alpha=alpha,
-                                        beta=beta, out=out)
+                                        beta=beta, out=out, prec=self.prec)
         return self.handle.backmul(M, psi, transM=not self.symmetric, alpha=alpha, beta=beta,
                                    out=out, prec=self.prec)
 
@@ -472,13 +474,14 @@ class PairCalculator:
                             beta: float = 0.0):
         """-F*T (ckPairCalculator.C:2018, alpha = -1)."""
         if self.complex_math:
-            return self.handle.zbackmul(orthoT, F, alpha=-1.0, beta=beta, out=out)
+            return self.handle.zbackmul(orthoT, F, alpha=-1.0, beta=beta, out=out,
+                                        prec=self.prec)
         return self.handle.backmul(orthoT, F, transM=False, alpha=-1.0, beta=beta, out=out,
                                    prec=self.prec)
 
     def multiplyPsiV(self, orthoT: torch.Tensor, psiV: torch.Tensor, out=None):
         if self.complex_math:
-            return self.handle.zbackmul(orthoT, psiV, out=out)
+            return self.handle.zbackmul(orthoT, psiV, out=out, prec=self.prec)
         return self.handle.backmul(orthoT, psiV, transM=False, out=out, prec=self.prec)
 
 

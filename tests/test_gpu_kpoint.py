@@ -154,3 +154,41 @@ def test_complex_pair_calculator_round_trip(handle, oracle):
     assert rel(L.cpu().numpy(), L_h) < 1e-13
     assert rel(G.cpu().numpy(), Tw @ L_h) < 1e-12
     assert rel(back.cpu().numpy(), oracle.zbackmul(L_h, psi_h, transM=True)) < 1e-12
+
+
+# ---- split precision --------------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["int8x5", "fp16x5"])
+def test_zortho_step_split_meets_residual(handle, prec):
+    """The complex step on digit products: S within 1e-10 of the FP64 step,
+    T psi orthonormal to 1e-10 (S->T is FP64 in every mode)."""
+    n, npw = 256, 8192
+    psi = dev(make_kpoint_states(n, npw, eps=1e-3, seed=13).numpy())
+    S64, T64, o64, it64, _ = handle.zortho_step(psi, nspin=2, tol=1e-13, maxit=50, modulus=True)
+    Ss, Ts, os_, its, _ = handle.zortho_step(psi, nspin=2, tol=1e-13, maxit=50, modulus=True,
+                                             prec=prec)
+    torch.cuda.synchronize()
+    eS = rel(Ss.cpu().numpy(), S64.cpu().numpy())
+    eO = rel(os_.cpu().numpy(), o64.cpu().numpy())
+    S2 = handle.zoverlap(os_)
+    resid = (S2 - torch.eye(n, dtype=S2.dtype, device=DEV)).abs().max().item()
+    print(f"{prec}: S err {eS:.2e}, T psi err {eO:.2e}, residual {resid:.2e}")
+    assert np.array_equal(Ss.cpu().numpy(), Ss.cpu().numpy().conj().T)
+    assert eS < 1e-10 and eO < 1e-10 and resid < 1e-10
+
+
+@pytest.mark.parametrize("transM", [False, True])
+def test_zbackmul_and_zlambda_split(handle, transM):
+    n, npw = 128, 4096
+    psi = dev(make_kpoint_states(n, npw, eps=0.0, seed=14).numpy())
+    F = dev(make_kpoint_states(n, npw, eps=1e-1, seed=15).numpy())
+    rng = np.random.default_rng(16)
+    M = dev(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    a = handle.zbackmul(M, psi, transM=transM, alpha=-0.5, prec="fp64")
+    b = handle.zbackmul(M, psi, transM=transM, alpha=-0.5, prec="int8x5")
+    La = handle.zlambda(psi, F, prec="fp64")
+    Lb = handle.zlambda(psi, F, prec="int8x5")
+    torch.cuda.synchronize()
+    # a general matrix gets no diagonal gain: the 4-slice truncation of its digits
+    assert rel(b.cpu().numpy(), a.cpu().numpy()) < 1e-6
+    assert rel(Lb.cpu().numpy(), La.cpu().numpy()) < 1e-9
