@@ -231,6 +231,31 @@ class GpuExecutor:
         self.record = record
         self.records: dict[int, list] = {}
 
+    def tune(self, sessions: int = 24, seed: int = 12345) -> float:
+        """Tune the batch's tile table once, before serving, on a synthetic
+        mix drawn from the serving size distribution; every later plan takes
+        its tile choices from that table.  Returns the seconds spent."""
+        from .workload import make_states
+        torch = self.torch
+        rng = np.random.default_rng(seed)
+        t0 = time.perf_counter()
+        psis = []
+        for i in range(sessions):
+            n = int(rng.choice(SIZES, p=WEIGHTS))
+            psis.append(make_states(n, 64 * n, eps=1e-3, seed=seed + i, device="cuda",
+                                    rng="torch"))
+        self.batch.plan(psis, nkx0=1, nspin=self.nspin, tol=self.tol, maxit=self.maxit,
+                        prec=self.prec)
+        self.batch.tune()
+        torch.cuda.synchronize()
+        # drop the mix: the next plan (the first real step) replaces the tables
+        self.batch._keep = []
+        self.batch.psis = self.batch.outs = self.batch.S = self.batch.T = None
+        self.members = ()
+        del psis
+        torch.cuda.empty_cache()
+        return time.perf_counter() - t0
+
     def footprint(self, n: int, npw: int) -> int:
         # psi, psi_out and the MD perturbation block, S and T, plus the batch
         # workspace share
