@@ -351,10 +351,21 @@ void ovl_f64_kernel(OvlArgs a) {
   __shared__ __attribute__((aligned(16))) double smem[4 * BM * LS];
   const int Tm = cdiv(a.M, BM), Tn = cdiv(a.N, BM);
   const int ntiles = SYM ? Tm * (Tm + 1) / 2 : Tm * Tn;
-  const int nz = a.nz_main + (a.kx0_end > 0 ? 1 : 0);
-  const int logical = xcd_logical(blockIdx.x, gridDim.x);
-  if (logical >= ntiles * nz) return;
-  const int tile = logical % ntiles, z = logical / ntiles;
+  // The short kx = 0 slice (z = 0) takes the leading workgroups, which the
+  // dispatcher spreads over all XCDs; the main chunks follow in XCD ranges.
+  const int hx = a.kx0_end > 0 ? 1 : 0;
+  const int nkx = hx ? cdiv(ntiles, 8) * 8 : 0;
+  int tile, z;
+  if ((int)blockIdx.x < nkx) {
+    if ((int)blockIdx.x >= ntiles) return;
+    tile = blockIdx.x;
+    z = 0;
+  } else {
+    const int logical = xcd_logical(blockIdx.x - nkx, gridDim.x - nkx);
+    if (logical >= ntiles * a.nz_main) return;
+    tile = logical % ntiles;
+    z = logical / ntiles + hx;
+  }
   int bi, bj;
   if (SYM) upper_tile(tile, Tm, bi, bj);
   else { bi = tile % Tm; bj = tile / Tm; }
@@ -647,7 +658,31 @@ struct OvlPlan {
   int WM, BM, ntiles, nz_main, kchunk, nz;
 };
 
-OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym, int nz_override = 0) {
+// Split depth nz for ntiles output tiles over K columns.  The main chunks go
+// to the 8 XCDs in equal contiguous ranges of cdiv(ntiles*nz, 8) workgroups,
+// of which 32 CUs x wpc are resident at once (wpc = 2 for WM = 4, whose two
+// 74 KB LDS footprints fill a CU's 160 KB; 4 for WM = 2), so a depth costs
+// rounds x cdiv(K, nz) columns of one workgroup's MFMA time (2 flops per
+// output element and column on 1/wpc of a CU's share of the FP64 MFMA peak:
+// 6.5 ps x wpc) plus writing and re-reading the nz*ntiles partial tiles
+// (16 bytes per element at about 5 TB/s: 3.2 ps); both terms are per output
+// element of a tile.  The cheapest depth wins; chunks keep at least 8 BK
+// stages.  The choice depends on (M, N, K) only, so workspace sizing and
+// launch agree for any kx = 0 split.
+int ovl_depth(int ntiles, int K, int wpc) {
+  const int nzmax = max(1, min(512, cdiv(K, 8 * BK)));
+  const long slots = 32L * wpc;
+  int best = 1;
+  double best_cost = 0.0;
+  for (int nz = 1; nz <= nzmax; nz++) {
+    const long rounds = (cdiv(ntiles * nz, 8) + slots - 1) / slots;
+    const double cost = (double)rounds * cdiv(K, nz) * wpc * 6.5 + (double)nz * ntiles * 3.2;
+    if (nz == 1 || cost < best_cost) { best = nz; best_cost = cost; }
+  }
+  return best;
+}
+
+OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym) {
   OvlPlan pl;
   const int t128 = sym ? cdiv(M, 128) * (cdiv(M, 128) + 1) / 2 : cdiv(M, 128) * cdiv(N, 128);
   pl.WM = (M >= 256 || t128 >= 16) ? 4 : 2;
@@ -655,11 +690,7 @@ OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym, int nz_override = 0
   const int Tm = cdiv(M, pl.BM), Tn = cdiv(N, pl.BM);
   pl.ntiles = sym ? Tm * (Tm + 1) / 2 : Tm * Tn;
   const int Kmain = K - kx0_end;
-  // about two waves of the 512 resident workgroups (256 CUs x 2), chunks of
-  // at least 8 stages; an autotuned depth (oa_tune_overlap) takes precedence
-  int nz = nz_override > 0 ? nz_override : cdiv(1024, pl.ntiles);
-  nz = min(nz, max(1, cdiv(Kmain, 8 * BK)));
-  nz = max(1, min(nz, 512));
+  int nz = ovl_depth(pl.ntiles, max(K, 1), pl.WM == 4 ? 2 : 4);
   int kchunk = cdiv(cdiv(max(Kmain, 1), nz), BK) * BK;
   nz = max(1, cdiv(max(Kmain, 1), kchunk));
   pl.nz_main = nz;
@@ -715,8 +746,8 @@ template <int WM, bool SYM>
 static int launch_ovl(hipStream_t s, const OvlPlan &pl, OvlArgs a, int M, int N, int kx0_end,
                       double alpha, double *C, long ldc, double diag,
                       unsigned long long *maxdev) {
-  const int G = pl.ntiles * pl.nz;
-  const int nb = cdiv(G, 8) * 8;
+  const int nkx = kx0_end > 0 ? cdiv(pl.ntiles, 8) * 8 : 0;
+  const int nb = nkx + cdiv(pl.ntiles * pl.nz_main, 8) * 8;
   hipLaunchKernelGGL((ovl_f64_kernel<WM, SYM>), dim3(nb), dim3(NT), 0, s, a);
   if (maxdev) hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, maxdev, 0ull, 1);
   const long total = (long)pl.ntiles * pl.BM * pl.BM;
