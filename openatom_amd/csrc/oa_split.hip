@@ -662,11 +662,19 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   const char *Xg = reinterpret_cast<const char *>(a.X);
   const char *Yg = reinterpret_cast<const char *>(a.Y);
   const size_t sx = a.sx, sy = a.sy, ldx = a.ldx, ldy = a.ldy;
+  const int nst = (kb - ka) / SK;
+  // MODE 1 with int8 digits contracts the tile's own ND row blocks last, so
+  // their NL slices are still in the ring when the epilogue rebuilds psi~
+  // (those blocks of DB are the psi digits of the tile's output rows).  The
+  // int32 level sums do not depend on the order.
+  constexpr int ND = TILE / SK;
+  constexpr bool DIAG_LDS = (MODE == 1) && (DT == DT_I8) && (ND <= NBUF);
+  const int rot = DIAG_LDS ? m0 / SK + ND : 0;
   // Wave-load i of a stage: operand o, slice p, quarter q (rows 32q..32q+31).
   // Lane L fills LDS slot q*64 + L of the panel, i.e. row r = (q*64+L)/2,
   // stored half hs = L%2, which holds global half hs ^ ((r >> 3) & 1).
   auto issue = [&](int st) {
-    const int kblk = (ka + st * SK) / SK;
+    const int kblk = DIAG_LDS ? (st + rot) % nst : (ka + st * SK) / SK;
     char *buf = smem + (st % NBUF) * STAGE;
 #pragma unroll
     for (int j = 0; j < PERW; j++) {
@@ -680,7 +688,6 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
     }
   };
-  const int nst = (kb - ka) / SK;
 #pragma unroll
   for (int st = 0; st < NBUF - 1; st++)
     if (st < nst) issue(st);
@@ -773,12 +780,24 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
         dv[r] = a.diagv[m];
         fe[r] = a.fexp[m];
       }
-      // psi~[m][nn] for these rows, rebuilt from all S digits
+      // psi~[m][nn] for these rows, rebuilt from all S digits; with DIAG_LDS
+      // the leading NL slices come from the ring: row block m0/SK + xr/SK was
+      // stage nst - ND + xr/SK, and the 4 rows are bytes b0..b0+3 of LDS row
+      // ln (16-byte halves swapped on odd groups of 8 rows, as stored)
+      const int b0 = 8 * g + 4 * (lane >> 5);
 #pragma unroll
       for (int p = S - 1; p >= 0; p--) {
         double q4[4];
-        load_digit_quad<DT>(a.DB, (size_t)p * a.dbs + kb_off<DT>(min(nn, a.N - 1), mg,
-                                                                   (size_t)a.Kp), q4);
+        if (DIAG_LDS && p < NL) {
+          const char *yb = smem + ((nst - ND + xr / SK) % NBUF) * STAGE + NL * PANEL + p * PANEL;
+          const int wv = *reinterpret_cast<const int *>(
+              yb + ln * 32 + (((b0 >> 4) ^ ((ln >> 3) & 1)) << 4) + (b0 & 15));
+#pragma unroll
+          for (int r = 0; r < 4; r++) q4[r] = (double)(int8_t)((wv >> (8 * r)) & 0xff);
+        } else {
+          load_digit_quad<DT>(a.DB, (size_t)p * a.dbs + kb_off<DT>(min(nn, a.N - 1), mg,
+                                                                     (size_t)a.Kp), q4);
+        }
 #pragma unroll
         for (int r = 0; r < 4; r++) pd[r] += q4[r] * rinv(p);
       }
