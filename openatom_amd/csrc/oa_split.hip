@@ -542,6 +542,17 @@ __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
+// Workgroup barrier of the LDS ring.  __syncthreads() carries a workgroup
+// fence, for which the compiler drains every vector memory operation of the
+// wave (vmcnt(0)) before s_barrier: the direct-to-LDS loads of the stages
+// meant to stay in flight would be waited for at every stage.  The ring only
+// needs the wave's own LDS reads finished (lgkmcnt(0)) before the barrier;
+// the loads of a stage are awaited with wait_vm before it is read.  The
+// memory clobber keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void ring_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Digits of DB for 4 consecutive contraction rows (one dword group for int8,
 // one 8-byte group for 16-bit digits).
 template <int DT>
@@ -698,7 +709,7 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
     else if (NBUF >= 4 && ahead >= 2) wait_vm<2 * PERW>();
     else if (ahead >= 1) wait_vm<PERW>();
     else wait_vm<0>();
-    __syncthreads();
+    ring_barrier();
     if (s + NBUF - 1 < nst) issue(s + NBUF - 1);
     const char *cur = smem + (s % NBUF) * STAGE;
     digit_stage<DT, NL, WI, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
@@ -827,8 +838,8 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       }
     }
   }
-  // the next item's first loads overwrite the ring
-  __syncthreads();
+  // the next item's first loads overwrite the ring (stores may stay in flight)
+  ring_barrier();
 }
 
 template <int DT, int S, int MODE, int NLB, int TILE>
