@@ -187,13 +187,15 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
   OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, phi, ldphi, J, (long)K);
   int st;
   if (is_split(prec)) {
+    // Im first: the symmetric overlap then leaves psi's digits (both layouts)
+    // in the kernel workspace, where a following back-multiply can reuse them
+    st = oa_launch_lambda_split(s, prec, n, K, psi, ld, J, K, alpha, Li, kws, kbytes);
+    if (st) return st;
     if (sym)
       st = oa_launch_overlap_split(s, prec, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws,
                                    kbytes);
     else
       st = oa_launch_lambda_split(s, prec, n, K, psi, ld, phi, ldphi, alpha, Lr, kws, kbytes);
-    if (st) return st;
-    st = oa_launch_lambda_split(s, prec, n, K, psi, ld, J, K, alpha, Li, kws, kbytes);
   } else {
     if (sym)
       st = oa_launch_overlap_f64(s, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws, kbytes);
@@ -211,10 +213,11 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
 }
 
 // out = alpha op(M) psi + beta out, op(M) = M or M^H; `J` holds J psi
-// already when have_j.
+// already when have_j; with reuse_psi the kernel workspace holds psi's
+// digits from a symmetric split forward on the same psi (no re-slicing).
 int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const double *psi,
               long ld, double alpha, double beta, double *out, long ldo, char *ws,
-              size_t ws_bytes, bool have_j, int prec) {
+              size_t ws_bytes, bool have_j, int prec, bool reuse_psi = false) {
   const int K = 2 * npw;
   double *J = reinterpret_cast<double *>(ws);
   double *Mr = reinterpret_cast<double *>(ws + off_planes(n, K));
@@ -226,8 +229,8 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
     char *kws = ws + off_kws(n, K);
     const size_t kbytes = ws_bytes - off_kws(n, K);
     const int p = prec & ~OA_PREC_REUSE_DIGITS;
-    int st = oa_launch_backmul_split(s, p, n, K, Mr, transM, psi, ld, 0, alpha, beta, out, ldo,
-                                     kws, kbytes);
+    int st = oa_launch_backmul_split(s, reuse_psi ? (p | OA_PREC_REUSE_DIGITS) : p, n, K, Mr,
+                                     transM, psi, ld, 0, alpha, beta, out, ldo, kws, kbytes);
     if (st) return st;
     return oa_launch_backmul_split(s, p, n, K, Mi, transM, J, K, 0, transM ? -alpha : alpha, 1.0,
                                    out, ldo, kws, kbytes);
@@ -403,9 +406,10 @@ int oa_zortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi,
   if (st) return st;
   const int st_ns = zs_to_t_run(c, n, S, nspin, tol, maxit, modulus, T, iters, err);
   if (st_ns != OA_SUCCESS && st_ns != OA_ERR_NOT_CONVERGED) return st_ns;
-  // the forward left J psi at the start of the workspace
+  // the forward left J psi at the start of the workspace and, in the split
+  // modes, psi's digits in the kernel workspace (S->T has buffers of its own)
   st = zbackward(c->stream, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, ws, c->ws.bytes,
-                 true, prec);
+                 true, prec, is_split(prec));
   return st ? st : st_ns;
 }
 

@@ -192,3 +192,19 @@ def test_zbackmul_and_zlambda_split(handle, transM):
     # a general matrix gets no diagonal gain: the 4-slice truncation of its digits
     assert rel(b.cpu().numpy(), a.cpu().numpy()) < 1e-6
     assert rel(Lb.cpu().numpy(), La.cpu().numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["int8x5", "int8x6l"])
+def test_zortho_step_split_matches_separate_calls(handle, prec):
+    """The fused split step reuses psi's digits from its overlap in the first
+    back-multiply; the separate calls slice psi again.  Same bits."""
+    n, npw = 160, 3000
+    psi = dev(make_kpoint_states(n, npw, eps=1e-3, seed=17).numpy())
+    S, T, out, it, _ = handle.zortho_step(psi, nspin=2, tol=1e-13, maxit=50, modulus=True,
+                                          prec=prec)
+    S2 = handle.zoverlap(psi, prec=prec)
+    T2, it2, _ = handle.zs_to_t(S2, nspin=2, tol=1e-13, maxit=50, modulus=True)
+    out2 = handle.zbackmul(T2, psi, prec=prec)
+    torch.cuda.synchronize()
+    assert it == it2
+    assert torch.equal(S, S2) and torch.equal(T, T2) and torch.equal(out, out2)
