@@ -134,6 +134,10 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K,
                             long ldpsi, int kx0_end, double alpha, double beta,
                             double *out, long ldout, void *ws, size_t ws_bytes,
                             int cw = 0, double alpha_w = 0.0);
+// In place on the back-multiply digits (DB) a split slicing of an [n][K]
+// block left in ws: digits of psi become those of J psi, J (re, im) =
+// (-im, re), exactly (the chunk maxima do not change).
+int oa_launch_jswap_db(hipStream_t s, int prec, int n, int K, void *ws, size_t ws_bytes);
 
 // ---------------------------------------------------------------- grouped
 // A grouped launch walks `total` work items spread over `count` instances:

@@ -232,8 +232,14 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
     int st = oa_launch_backmul_split(s, reuse_psi ? (p | OA_PREC_REUSE_DIGITS) : p, n, K, Mr,
                                      transM, psi, ld, 0, alpha, beta, out, ldo, kws, kbytes);
     if (st) return st;
-    return oa_launch_backmul_split(s, p, n, K, Mi, transM, J, K, 0, transM ? -alpha : alpha, 1.0,
-                                   out, ldo, kws, kbytes);
+    if (reuse_psi) {
+      // J psi's digits from psi's, in place, instead of slicing J psi
+      st = oa_launch_jswap_db(s, p, n, K, kws, kbytes);
+      if (st) return st;
+    }
+    return oa_launch_backmul_split(s, reuse_psi ? (p | OA_PREC_REUSE_DIGITS) : p, n, K, Mi,
+                                   transM, J, K, 0, transM ? -alpha : alpha, 1.0, out, ldo, kws,
+                                   kbytes);
   }
   oa_gemm_problem p{n, K, n, Mr, n, psi, ld, out, ldo, alpha, beta, 0.0};
   int st = oa_launch_gemm_f64(s, transM != 0, false, &p, 1, nullptr);

@@ -1282,6 +1282,36 @@ int dispatch_backmul(hipStream_t st, const oa_split_plan &pl, bool lean, int n, 
   return OA_ERR_UNSUPPORTED;
 }
 
+// In place: psi's digits in the back-multiply layout become J psi's, J (re,
+// im) = (-im, re).  Columns 2c and 2c+1 of one 32-byte contraction block are
+// adjacent 32-byte rows of DB, so every 64-byte unit has its halves swapped
+// and its new first half negated.  The digits of -x are the negated digits
+// of x (rint is odd), and a chunk holds whole (re, im) pairs, so the chunk
+// maxima stay valid.  Thread t takes 16 bytes of each half of unit t / 2.
+template <int DT>
+__device__ __forceinline__ i4v neg_digits(i4v v) {
+  if constexpr (DT == DT_I8) {
+    typedef signed char c16 __attribute__((ext_vector_type(16)));
+    return __builtin_bit_cast(i4v, -__builtin_bit_cast(c16, v));
+  } else if constexpr (DT == DT_F32) {
+    return v ^ (int)0x80000000;
+  } else {
+    return v ^ (int)0x80008000;
+  }
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void jswap_db_kernel(char *DB, long units) {
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < 2 * units;
+       t += (long)gridDim.x * 256) {
+    char *u = DB + (t >> 1) * 64 + (t & 1) * 16;
+    const i4v a = *reinterpret_cast<const i4v *>(u);
+    const i4v b = *reinterpret_cast<const i4v *>(u + 32);
+    *reinterpret_cast<i4v *>(u) = neg_digits<DT>(b);
+    *reinterpret_cast<i4v *>(u + 32) = a;
+  }
+}
+
 // ------------------------------------------------------------ grouped launches
 template <int DT, int S, int NLB, int TILE>
 int grouped_phase(hipStream_t s, int phase, const oa_group &g) {
@@ -1405,6 +1435,24 @@ int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *
   if (dt == DT_F16) return dispatch_lambda<DT_F16>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
   if (dt == DT_F32) return dispatch_lambda<DT_F32>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
   return dispatch_lambda<DT_BF16>(s, lp, n, K, psi, ld, F, ldF, alpha, L, w);
+}
+
+int oa_launch_jswap_db(hipStream_t s, int prec, int n, int K, void *ws, size_t ws_bytes) {
+  int dt, Sl;
+  if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
+  const oa_split_plan pl = make_plan(prec & 0xffff, n, K, 128);
+  if (pl.total > ws_bytes) return OA_ERR_CAPACITY;
+  // DB: S slices of Kp x np digits; Kp and np are multiples of 128, so whole
+  // 64-byte units
+  const long units = (long)pl.S * pl.Kp * pl.np * pl.es / 64;
+  if (units <= 0) return OA_SUCCESS;
+  const unsigned g = (unsigned)std::min<long>((2 * units + 255) / 256, 8192);
+  char *db = (char *)ws + pl.off_db;
+  if (dt == DT_I8) hipLaunchKernelGGL(jswap_db_kernel<DT_I8>, dim3(g), dim3(256), 0, s, db, units);
+  else if (dt == DT_F32)
+    hipLaunchKernelGGL(jswap_db_kernel<DT_F32>, dim3(g), dim3(256), 0, s, db, units);
+  else hipLaunchKernelGGL(jswap_db_kernel<DT_F16>, dim3(g), dim3(256), 0, s, db, units);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double *M, int transM,

@@ -194,10 +194,12 @@ def test_zbackmul_and_zlambda_split(handle, transM):
     assert rel(Lb.cpu().numpy(), La.cpu().numpy()) < 1e-9
 
 
-@pytest.mark.parametrize("prec", ["int8x5", "int8x6l"])
+@pytest.mark.parametrize("prec", ["int8x5", "int8x6l", "fp16x5", "bf16x5"])
 def test_zortho_step_split_matches_separate_calls(handle, prec):
     """The fused split step reuses psi's digits from its overlap in the first
-    back-multiply; the separate calls slice psi again.  Same bits."""
+    back-multiply and turns them into J psi's in place (pair swap, negation)
+    for the second; the separate calls slice psi and J psi from FP64.  Same
+    bits."""
     n, npw = 160, 3000
     psi = dev(make_kpoint_states(n, npw, eps=1e-3, seed=17).numpy())
     S, T, out, it, _ = handle.zortho_step(psi, nspin=2, tol=1e-13, maxit=50, modulus=True,
