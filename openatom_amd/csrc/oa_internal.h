@@ -138,6 +138,12 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K,
 // block left in ws: digits of psi become those of J psi, J (re, im) =
 // (-im, re), exactly (the chunk maxima do not change).
 int oa_launch_jswap_db(hipStream_t s, int prec, int n, int K, void *ws, size_t ws_bytes);
+// Im S = alpha psi (J psi)^T from the digits a split overlap of the same
+// [n][K] block without kx = 0 columns left in ws (no slicing); needs
+// oa_split_zimag_ws_bytes, which includes the overlap's own workspace.
+size_t oa_split_zimag_ws_bytes(int prec, int n, int K);
+int oa_launch_zimag_split(hipStream_t s, int prec, int n, int K, double alpha, double *Li,
+                          void *ws, size_t ws_bytes);
 
 // ---------------------------------------------------------------- grouped
 // A grouped launch walks `total` work items spread over `count` instances:

@@ -164,6 +164,7 @@ size_t zoverlap_ws_bytes(int n, int K, int prec) {
   if (is_split(prec)) {
     if (oa_split_ws_bytes(prec, n, K) > kws) kws = oa_split_ws_bytes(prec, n, K);
     if (oa_split_lambda_ws_bytes(prec, n, K) > kws) kws = oa_split_lambda_ws_bytes(prec, n, K);
+    if (oa_split_zimag_ws_bytes(prec, n, K) > kws) kws = oa_split_zimag_ws_bytes(prec, n, K);
   }
   return off_kws(n, K) + kws;
 }
@@ -187,15 +188,19 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
   OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, phi, ldphi, J, (long)K);
   int st;
   if (is_split(prec)) {
-    // Im first: the symmetric overlap then leaves psi's digits (both layouts)
-    // in the kernel workspace, where a following back-multiply can reuse them
-    st = oa_launch_lambda_split(s, prec, n, K, psi, ld, J, K, alpha, Li, kws, kbytes);
-    if (st) return st;
-    if (sym)
+    if (sym) {
+      // psi is sliced once: Re S on its digits, Im S on them and on J psi's
+      // made from them; psi's digits (both layouts) stay in the kernel
+      // workspace, where a following back-multiply can reuse them
       st = oa_launch_overlap_split(s, prec, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws,
                                    kbytes);
-    else
+      if (st) return st;
+      st = oa_launch_zimag_split(s, prec, n, K, alpha, Li, kws, kbytes);
+    } else {
+      st = oa_launch_lambda_split(s, prec, n, K, psi, ld, J, K, alpha, Li, kws, kbytes);
+      if (st) return st;
       st = oa_launch_lambda_split(s, prec, n, K, psi, ld, phi, ldphi, alpha, Lr, kws, kbytes);
+    }
   } else {
     if (sym)
       st = oa_launch_overlap_f64(s, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws, kbytes);

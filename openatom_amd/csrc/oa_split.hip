@@ -1300,6 +1300,31 @@ __device__ __forceinline__ i4v neg_digits(i4v v) {
   }
 }
 
+// The same map on the S layout, out of place: there columns 2c and 2c+1 are
+// adjacent elements of a 32-byte row, so every 16 bytes hold whole pairs.
+template <int DT>
+__device__ __forceinline__ i4v jpairs(i4v v) {
+  if constexpr (DT == DT_I8) {
+    typedef signed char c16 __attribute__((ext_vector_type(16)));
+    const c16 x = __builtin_bit_cast(c16, v), m = -x;
+    return __builtin_bit_cast(i4v, __builtin_shufflevector(x, m, 17, 0, 19, 2, 21, 4, 23, 6, 25,
+                                                           8, 27, 10, 29, 12, 31, 14));
+  } else if constexpr (DT == DT_F32) {
+    const i4v m = v ^ (int)0x80000000;
+    return __builtin_shufflevector(v, m, 5, 0, 7, 2);
+  } else {
+    typedef short s8v __attribute__((ext_vector_type(8)));
+    const s8v x = __builtin_bit_cast(s8v, v), m = x ^ (short)0x8000;
+    return __builtin_bit_cast(i4v, __builtin_shufflevector(x, m, 9, 0, 11, 2, 13, 4, 15, 6));
+  }
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void jswap_ds_kernel(const char *src, char *dst, long chunks) {
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < chunks; t += (long)gridDim.x * 256)
+    *reinterpret_cast<i4v *>(dst + t * 16) = jpairs<DT>(*reinterpret_cast<const i4v *>(src + t * 16));
+}
+
 template <int DT>
 __global__ __launch_bounds__(256) void jswap_db_kernel(char *DB, long units) {
   for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < 2 * units;
@@ -1453,6 +1478,92 @@ int oa_launch_jswap_db(hipStream_t s, int prec, int n, int K, void *ws, size_t w
     hipLaunchKernelGGL(jswap_db_kernel<DT_F32>, dim3(g), dim3(256), 0, s, db, units);
   else hipLaunchKernelGGL(jswap_db_kernel<DT_F16>, dim3(g), dim3(256), 0, s, db, units);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+// Im S = alpha psi (J psi)^T on the digits a split overlap of the same
+// [n][K] block (no kx = 0 columns) left in ws: J psi's S-layout digits are
+// made from psi's (jswap_ds_kernel) in the area after the overlap plan, and
+// the asymmetric digit product (MODE 2, all tiles) and its fixed-order
+// reduce run as in the split Lambda, whose slicing would give the same
+// digits and chunk maxima.
+struct ZimagPlan {
+  oa_split_plan pl;
+  int ntiles;
+  size_t off_yj, off_part, total;
+};
+
+ZimagPlan make_zimag_plan(int prec, int n, int K) {
+  ZimagPlan zp;
+  zp.pl = make_plan(prec, n, K, 128);
+  const oa_split_plan &pl = zp.pl;
+  const int T = pl.np / 128;
+  zp.ntiles = T * T;
+  zp.off_yj = (pl.total + 255) / 256 * 256;
+  zp.off_part = zp.off_yj + ((size_t)pl.S * pl.np * pl.Kp * pl.es + 255) / 256 * 256;
+  zp.total = zp.off_part + (size_t)pl.nz * zp.ntiles * 128 * 128 * 8;
+  return zp;
+}
+
+template <int DT, int S>
+int run_zimag(hipStream_t s, const ZimagPlan &zp, int n, double alpha, double *Li, char *ws) {
+  const oa_split_plan &pl = zp.pl;
+  const long chunks = (long)pl.S * pl.np * pl.Kp * pl.es / 16;
+  const unsigned g = (unsigned)std::min<long>((chunks + 255) / 256, 8192);
+  hipLaunchKernelGGL(jswap_ds_kernel<DT>, dim3(g), dim3(256), 0, s, ws + pl.off_ds,
+                     ws + zp.off_yj, chunks);
+  auto *mS = reinterpret_cast<unsigned long long *>(ws + pl.off_maxS);
+  oa_digit_args a{};
+  a.X = ws + pl.off_ds;
+  a.Y = ws + zp.off_yj;
+  a.ldx = a.ldy = pl.np;
+  a.sx = a.sy = (long)pl.np * pl.Kp;
+  a.M = n;
+  a.N = n;
+  a.kc = pl.kchunk;
+  a.kfull = pl.Kp;
+  a.ntiles = zp.ntiles;
+  a.nz = pl.nz;
+  a.part = reinterpret_cast<double *>(ws + zp.off_part);
+  a.chunk_max = mS;
+  a.colmax = mS;
+  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 2, S - 1, 128>),
+                  (DigitCfg<DT, S, 2, S - 1, 128>::NTHR), zp.ntiles * pl.nz, s, a);
+  const long total = (long)zp.ntiles * 128 * 128;
+  hipLaunchKernelGGL(lambda_reduce_kernel<128>, dim3((unsigned)cdv((int)total, 256)), dim3(256),
+                     0, s, (const double *)a.part, n, pl.nz, alpha, Li);
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
+}
+
+template <int DT>
+int dispatch_zimag(hipStream_t st, const ZimagPlan &zp, int n, double alpha, double *Li,
+                   char *ws) {
+  switch (zp.pl.S) {
+    case 2: return run_zimag<DT, 2>(st, zp, n, alpha, Li, ws);
+    case 3: return run_zimag<DT, 3>(st, zp, n, alpha, Li, ws);
+    case 4: return run_zimag<DT, 4>(st, zp, n, alpha, Li, ws);
+    case 5: return run_zimag<DT, 5>(st, zp, n, alpha, Li, ws);
+    case 6: return run_zimag<DT, 6>(st, zp, n, alpha, Li, ws);
+  }
+  return OA_ERR_UNSUPPORTED;
+}
+
+size_t oa_split_zimag_ws_bytes(int prec, int n, int K) {
+  int dt, Sl;
+  if (!decode_prec(prec & 0xffff, dt, Sl)) return 0;
+  return make_zimag_plan(prec & 0xffff, n, K).total;
+}
+
+int oa_launch_zimag_split(hipStream_t s, int prec, int n, int K, double alpha, double *Li,
+                          void *ws, size_t ws_bytes) {
+  int dt, Sl;
+  if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
+  const ZimagPlan zp = make_zimag_plan(prec & 0xffff, n, K);
+  if (zp.total > ws_bytes) return OA_ERR_CAPACITY;
+  char *w = (char *)ws;
+  if (dt == DT_I8) return dispatch_zimag<DT_I8>(s, zp, n, alpha, Li, w);
+  if (dt == DT_F16) return dispatch_zimag<DT_F16>(s, zp, n, alpha, Li, w);
+  if (dt == DT_F32) return dispatch_zimag<DT_F32>(s, zp, n, alpha, Li, w);
+  return dispatch_zimag<DT_BF16>(s, zp, n, alpha, Li, w);
 }
 
 int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double *M, int transM,

@@ -210,3 +210,18 @@ def test_zortho_step_split_matches_separate_calls(handle, prec):
     torch.cuda.synchronize()
     assert it == it2
     assert torch.equal(S, S2) and torch.equal(T, T2) and torch.equal(out, out2)
+
+
+@pytest.mark.parametrize("prec", ["int8x5", "fp16x5"])
+def test_zoverlap_split_imag_from_overlap_digits(handle, prec):
+    """The split Hermitian overlap takes Im S from the digits of its own Re S
+    slicing (J psi's made by a pair swap); the general product slices psi and
+    J psi from FP64.  The imaginary parts agree bit for bit."""
+    n, npw = 200, 2500
+    psi = dev(make_kpoint_states(n, npw, eps=1e-3, seed=19).numpy())
+    S = handle.zoverlap(psi, prec=prec)
+    L = handle.zlambda(psi, psi, prec=prec)
+    torch.cuda.synchronize()
+    iu = torch.triu_indices(n, n, device=DEV)
+    assert torch.equal(S.imag[iu[0], iu[1]], L.imag[iu[0], iu[1]])
+    assert not S.imag.diagonal().any().item()
