@@ -1482,10 +1482,13 @@ int oa_launch_jswap_db(hipStream_t s, int prec, int n, int K, void *ws, size_t w
 
 // Im S = alpha psi (J psi)^T on the digits a split overlap of the same
 // [n][K] block (no kx = 0 columns) left in ws: J psi's S-layout digits are
-// made from psi's (jswap_ds_kernel) in the area after the overlap plan, and
-// the asymmetric digit product (MODE 2, all tiles) and its fixed-order
-// reduce run as in the split Lambda, whose slicing would give the same
-// digits and chunk maxima.
+// made from psi's (jswap_ds_kernel) in the area after the overlap plan; the
+// split Lambda's slicing would give the same digits and chunk maxima.  Im S
+// is exactly antisymmetric in digit arithmetic and a Hermitian S needs only
+// its strict upper triangle, so the product runs on the upper tiles (the
+// symmetric kernel's tile map, X = psi, Y = J psi, both exponents psi's)
+// and its fixed-order reduce writes the upper triangle of Li as the
+// all-tiles product would (the mirrored lower triangle is not Im S).
 struct ZimagPlan {
   oa_split_plan pl;
   int ntiles;
@@ -1497,7 +1500,7 @@ ZimagPlan make_zimag_plan(int prec, int n, int K) {
   zp.pl = make_plan(prec, n, K, 128);
   const oa_split_plan &pl = zp.pl;
   const int T = pl.np / 128;
-  zp.ntiles = T * T;
+  zp.ntiles = T * (T + 1) / 2;
   zp.off_yj = (pl.total + 255) / 256 * 256;
   zp.off_part = zp.off_yj + ((size_t)pl.S * pl.np * pl.Kp * pl.es + 255) / 256 * 256;
   zp.total = zp.off_part + (size_t)pl.nz * zp.ntiles * 128 * 128 * 8;
@@ -1525,12 +1528,10 @@ int run_zimag(hipStream_t s, const ZimagPlan &zp, int n, double alpha, double *L
   a.nz = pl.nz;
   a.part = reinterpret_cast<double *>(ws + zp.off_part);
   a.chunk_max = mS;
-  a.colmax = mS;
-  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 2, S - 1, 128>),
-                  (DigitCfg<DT, S, 2, S - 1, 128>::NTHR), zp.ntiles * pl.nz, s, a);
-  const long total = (long)zp.ntiles * 128 * 128;
-  hipLaunchKernelGGL(lambda_reduce_kernel<128>, dim3((unsigned)cdv((int)total, 256)), dim3(256),
-                     0, s, (const double *)a.part, n, pl.nz, alpha, Li);
+  OA_LAUNCH_ITEMS((digit_nt_kernel<DT, S, 0, S - 1, 128>),
+                  (DigitCfg<DT, S, 0, S - 1, 128>::NTHR), zp.ntiles * pl.nz, s, a);
+  const oa_sreduce_args rd{a.part, n, pl.nz, alpha, nullptr, 0, 0, 0, Li, 0.0, nullptr};
+  OA_LAUNCH_ITEMS(split_reduce_kernel<128>, 256, cdv(zp.ntiles * 128 * 128, 256), s, rd);
   return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
