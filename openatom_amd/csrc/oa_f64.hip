@@ -344,7 +344,10 @@ __device__ __forceinline__ void upper_tile(int tile, int T, int &bi, int &bj) {
   bj = b + rem;
 }
 
-template <int WM, bool SYM>
+// XY (with SYM): the upper-tile map of the symmetric product over two
+// distinct operands X and Y (the complex overlap's Im S = psi (J psi)^T,
+// whose strict upper triangle is all a Hermitian S needs).
+template <int WM, bool SYM, bool XY = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void ovl_f64_kernel(OvlArgs a) {
   constexpr int BM = 32 * WM;
@@ -383,12 +386,12 @@ void ovl_f64_kernel(OvlArgs a) {
 #pragma unroll
     for (int j = 0; j < WM; j++) acc[i][j] = d4{0, 0, 0, 0};
   const int m0 = bi * BM, n0 = bj * BM;
-  const double *Yp = SYM ? a.X : a.Y;
-  const long ldy = SYM ? a.ldx : a.ldy;
+  const double *Yp = (SYM && !XY) ? a.X : a.Y;
+  const long ldy = (SYM && !XY) ? a.ldx : a.ldy;
   const bool fast = m0 + BM <= a.M && n0 + BM <= a.N && aligned16(a.X) && aligned16(Yp) &&
                     (a.ldx % 2) == 0 && (ldy % 2) == 0 && (ka % 2) == 0;
   if (ka < kb) {
-    if (SYM && bi == bj)
+    if (SYM && !XY && bi == bj)
       kloop<WM, true, true, true>(a.X, a.ldx, m0, a.M, a.X, a.ldx, n0, a.N, ka, kb, fast, smem,
                                   acc);
     else
@@ -742,13 +745,13 @@ size_t oa_ntsplit_f64_ws_bytes(int M, int N, int K) {
   return (size_t)pl.nz * pl.ntiles * pl.BM * pl.BM * sizeof(double) + 256;
 }
 
-template <int WM, bool SYM>
+template <int WM, bool SYM, bool XY = false>
 static int launch_ovl(hipStream_t s, const OvlPlan &pl, OvlArgs a, int M, int N, int kx0_end,
                       double alpha, double *C, long ldc, double diag,
                       unsigned long long *maxdev) {
   const int nkx = kx0_end > 0 ? cdiv(pl.ntiles, 8) * 8 : 0;
   const int nb = nkx + cdiv(pl.ntiles * pl.nz_main, 8) * 8;
-  hipLaunchKernelGGL((ovl_f64_kernel<WM, SYM>), dim3(nb), dim3(NT), 0, s, a);
+  hipLaunchKernelGGL((ovl_f64_kernel<WM, SYM, XY>), dim3(nb), dim3(NT), 0, s, a);
   if (maxdev) hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, maxdev, 0ull, 1);
   const long total = (long)pl.ntiles * pl.BM * pl.BM;
   hipLaunchKernelGGL((ovl_reduce_kernel<WM, SYM>), dim3((unsigned)cdiv((int)total, NT)),
@@ -786,6 +789,21 @@ int oa_launch_ntsplit_f64(hipStream_t s, int M, int N, int K, const double *X, l
   a.part = (double *)ws;
   if (pl.WM == 4) return launch_ovl<4, false>(s, pl, a, M, N, 0, alpha, C, ldc, 0.0, nullptr);
   return launch_ovl<2, false>(s, pl, a, M, N, 0, alpha, C, ldc, 0.0, nullptr);
+}
+
+int oa_launch_overlap_xy_f64(hipStream_t s, int n, int K, const double *X, long ldx,
+                             const double *Y, long ldy, double alpha, double *C, void *ws,
+                             size_t ws_bytes) {
+  const OvlPlan pl = plan_ovl(n, n, K, 0, true);
+  const size_t need = (size_t)pl.nz * pl.ntiles * pl.BM * pl.BM * sizeof(double);
+  if (need > ws_bytes) return OA_ERR_CAPACITY;
+  OvlArgs a;
+  a.X = X; a.ldx = ldx; a.Y = Y; a.ldy = ldy;
+  a.M = n; a.N = n; a.K = K;
+  a.kx0_end = 0; a.nz_main = pl.nz_main; a.kchunk = pl.kchunk;
+  a.part = (double *)ws;
+  if (pl.WM == 4) return launch_ovl<4, true, true>(s, pl, a, n, n, 0, alpha, C, n, 0.0, nullptr);
+  return launch_ovl<2, true, true>(s, pl, a, n, n, 0, alpha, C, n, 0.0, nullptr);
 }
 
 size_t oa_ns_ws_bytes(int n) {

@@ -94,6 +94,12 @@ size_t oa_ntsplit_f64_ws_bytes(int M, int N, int K);
 int oa_launch_ntsplit_f64(hipStream_t s, int M, int N, int K, const double *X,
                           long ldx, const double *Y, long ldy, double alpha,
                           double *C, long ldc, void *ws, size_t ws_bytes);
+// Upper tiles of C = alpha X Y^T (n x n) for two distinct [n][K] operands,
+// the rest of C the mirror (the strict upper triangle is the product); same
+// workspace as oa_launch_overlap_f64.
+int oa_launch_overlap_xy_f64(hipStream_t s, int n, int K, const double *X, long ldx,
+                             const double *Y, long ldy, double alpha, double *C, void *ws,
+                             size_t ws_bytes);
 
 // S->T on the device.  Workspace: oa_ns_ws_bytes(n).  begin / iterate /
 // finish let a host loop stop early; oa_launch_s_to_t enqueues `maxit`

@@ -202,12 +202,16 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
       st = oa_launch_lambda_split(s, prec, n, K, psi, ld, phi, ldphi, alpha, Lr, kws, kbytes);
     }
   } else {
-    if (sym)
+    if (sym) {
       st = oa_launch_overlap_f64(s, n, K, psi, ld, 0, alpha, Lr, 0.0, nullptr, kws, kbytes);
-    else
+      if (st) return st;
+      // the Hermitian merge reads only Im S's strict upper triangle
+      st = oa_launch_overlap_xy_f64(s, n, K, psi, ld, J, K, alpha, Li, kws, kbytes);
+    } else {
       st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, phi, ldphi, alpha, Lr, n, kws, kbytes);
-    if (st) return st;
-    st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, J, K, alpha, Li, n, kws, kbytes);
+      if (st) return st;
+      st = oa_launch_ntsplit_f64(s, n, n, K, psi, ld, J, K, alpha, Li, n, kws, kbytes);
+    }
   }
   if (st) return st;
   if (sym)
