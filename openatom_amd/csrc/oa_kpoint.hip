@@ -185,7 +185,10 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
   double *Li = reinterpret_cast<double *>(ws + off_planes(n, K) + plane_bytes(n));
   char *kws = ws + off_kws(n, K);
   const size_t kbytes = ws_bytes - off_kws(n, K);
-  OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, phi, ldphi, J, (long)K);
+  // the split Hermitian forward takes J psi's digits from psi's; every other
+  // form multiplies J phi in FP64 or slices it
+  if (!(sym && is_split(prec)))
+    OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, phi, ldphi, J, (long)K);
   int st;
   if (is_split(prec)) {
     if (sym) {
@@ -421,8 +424,10 @@ int oa_zortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi,
   if (st) return st;
   const int st_ns = zs_to_t_run(c, n, S, nspin, tol, maxit, modulus, T, iters, err);
   if (st_ns != OA_SUCCESS && st_ns != OA_ERR_NOT_CONVERGED) return st_ns;
-  // the forward left J psi at the start of the workspace and, in the split
-  // modes, psi's digits in the kernel workspace (S->T has buffers of its own)
+  // the forward left J psi at the start of the workspace (FP64) or psi's
+  // digits in the kernel workspace (split modes; S->T has buffers of its
+  // own), from which the back-multiply makes J psi's: J psi itself is not
+  // read there
   st = zbackward(c->stream, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, ws, c->ws.bytes,
                  true, prec, is_split(prec));
   return st ? st : st_ns;
