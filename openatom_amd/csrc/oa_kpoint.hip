@@ -234,7 +234,9 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
   double *J = reinterpret_cast<double *>(ws);
   double *Mr = reinterpret_cast<double *>(ws + off_planes(n, K));
   double *Mi = reinterpret_cast<double *>(ws + off_planes(n, K) + plane_bytes(n));
-  if (!have_j) OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, psi, ld, J, (long)K);
+  // split modes make J psi's digits from psi's and never read J psi
+  if (!have_j && !is_split(prec))
+    OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, psi, ld, J, (long)K);
   OA_KLAUNCH(zplanes_kernel, (long)n * n, s, (long)n * n, M, 1.0, Mr, Mi);
   // Re(M^H) = Mr^T and Im(M^H) = -Mi^T
   if (is_split(prec)) {
@@ -244,14 +246,12 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
     int st = oa_launch_backmul_split(s, reuse_psi ? (p | OA_PREC_REUSE_DIGITS) : p, n, K, Mr,
                                      transM, psi, ld, 0, alpha, beta, out, ldo, kws, kbytes);
     if (st) return st;
-    if (reuse_psi) {
-      // J psi's digits from psi's, in place, instead of slicing J psi
-      st = oa_launch_jswap_db(s, p, n, K, kws, kbytes);
-      if (st) return st;
-    }
-    return oa_launch_backmul_split(s, reuse_psi ? (p | OA_PREC_REUSE_DIGITS) : p, n, K, Mi,
-                                   transM, J, K, 0, transM ? -alpha : alpha, 1.0, out, ldo, kws,
-                                   kbytes);
+    // J psi's digits from psi's (left by the first product), in place,
+    // instead of slicing J psi; the second product reuses them
+    st = oa_launch_jswap_db(s, p, n, K, kws, kbytes);
+    if (st) return st;
+    return oa_launch_backmul_split(s, p | OA_PREC_REUSE_DIGITS, n, K, Mi, transM, J, K, 0,
+                                   transM ? -alpha : alpha, 1.0, out, ldo, kws, kbytes);
   }
   oa_gemm_problem p{n, K, n, Mr, n, psi, ld, out, ldo, alpha, beta, 0.0};
   int st = oa_launch_gemm_f64(s, transM != 0, false, &p, 1, nullptr);

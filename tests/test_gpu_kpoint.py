@@ -225,3 +225,24 @@ def test_zoverlap_split_imag_from_overlap_digits(handle, prec):
     iu = torch.triu_indices(n, n, device=DEV)
     assert torch.equal(S.imag[iu[0], iu[1]], L.imag[iu[0], iu[1]])
     assert not S.imag.diagonal().any().item()
+
+
+@pytest.mark.parametrize("prec", ["int8x5", "fp16x5"])
+@pytest.mark.parametrize("transM", [False, True])
+def test_zbackmul_split_equals_two_real_products(handle, prec, transM):
+    """The split complex back-multiply slices psi once and makes J psi's
+    digits from psi's in place; the real back-multiply of an explicit J psi
+    slices J psi from FP64.  Re(op M) psi + Im(op M) J psi, same bits."""
+    n, npw = 96, 2000
+    rng = np.random.default_rng(21)
+    psi_h = make_kpoint_states(n, npw, eps=0.0, seed=21).numpy()
+    j_h = np.empty_like(psi_h)
+    j_h[:, 0::2], j_h[:, 1::2] = -psi_h[:, 1::2], psi_h[:, 0::2]
+    M_h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    got = handle.zbackmul(dev(M_h), dev(psi_h), transM=transM, alpha=0.5, prec=prec)
+    # op(M) = M^H: Re = Mr^T, Im = -Mi^T
+    want = handle.backmul(dev(M_h.real), dev(psi_h), transM=transM, alpha=0.5, prec=prec)
+    want = handle.backmul(dev(M_h.imag), dev(j_h), transM=transM, alpha=-0.5 if transM else 0.5,
+                          beta=1.0, out=want, prec=prec)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
