@@ -86,8 +86,8 @@ def calibrate_copy():
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (separate runs) and scaled by
 # the same counters on a 1 GiB device copy (fetch x2.000, write x1.000).
 TRAFFIC = {
-    "fp64": 12.87e9 + 4.29e9,     # gemm_f64_kernel T*psi: read 12.87 GB, write 4.29 GB
-    "int8x5": 8.77e9 + 4.29e9,    # digit back-multiply: read 8.77 GB, write 4.29 GB
+    "fp64": 35.80e9 + 4.29e9,     # gemm_f64_kernel T*psi: read 35.80 GB, write 4.29 GB
+    "int8x5": 14.26e9 + 4.29e9,   # digit back-multiply: read 14.26 GB, write 4.29 GB
 }
 
 
