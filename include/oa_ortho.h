@@ -62,9 +62,22 @@ extern "C" {
 #define OA_PREC_FP32 4
 #define OA_PREC_SLICES(p, s) ((p) | ((s) << 8))
 #define OA_PREC_LEAN_BACKMUL (1 << 12)
-/* oa_backmul only: reuse the digit slices of psi left in the handle's
- * workspace by the preceding oa_overlap_sym of the same psi and precision
- * (the caller guarantees psi is unchanged in between). */
+/* Adaptive slice counts (OA_PREC_INT8 with 4 <= s <= 6): every K chunk z
+ * gets its own count s_z in [2, s], chosen on the device from the chunk
+ * maxima alone.  With W_z = K_z 4^(e_z) (K_z columns, chunk exponent e_z)
+ * and W = sum_z W_z, chunk z drops d slices when W_z 254^d <= W / nz.  The
+ * element-wise truncation bound of S (and of the residual after T psi) is
+ * at most twice the bound of s uniform slices; chunks whose coefficients are
+ * small next to the largest chunk run 2-4 slices instead of s.  Applies to
+ * oa_overlap_sym, oa_backmul, oa_lambda_force, oa_ortho_step and batches;
+ * the Lambda product and the complex calls return OA_ERR_UNSUPPORTED. */
+#define OA_PREC_ADAPTIVE (1 << 13)
+/* oa_backmul / oa_lambda_force: reuse the digit slices of psi left in the
+ * handle's workspace by the preceding split slicing on this handle (an
+ * oa_overlap_sym or a back-multiply of the same psi).  A hint: when the
+ * workspace no longer holds digits of this psi pointer, ld, n, npw and
+ * precision (any other call that used the workspace in between), psi is
+ * sliced again.  The caller guarantees psi's contents are unchanged. */
 #define OA_PREC_REUSE_DIGITS (1 << 16)
 
 typedef struct oa_context *oa_handle_t;
@@ -148,6 +161,16 @@ int oa_zgemm(oa_handle_t h, char transa, char transb, int m, int n, int k,
 int oa_overlap_sym(oa_handle_t h, int n, int npw, const double *psi,
                    long ldpsi, int nkx0, double alpha, int prec, double *S,
                    double diag, double *maxdev);
+
+/* The per chunk digit counts an adaptive split precision (OA_PREC_ADAPTIVE)
+ * assigns to psi: chunk maxima and the count rule on the device, then a
+ * synchronous copy of nchunks ints into counts (host).  *nchunks and *kchunk
+ * (columns per chunk, in doubles) are set even when capacity is too small
+ * (OA_ERR_CAPACITY).  No counterpart in the reference: it reports what the
+ * split chose (tests, benchmark histograms). */
+int oa_slice_counts(oa_handle_t h, int n, int npw, const double *psi,
+                    long ldpsi, int nkx0, int prec, int *counts, int capacity,
+                    int *nchunks, int *kchunk);
 
 /* Asymmetric PairCalculator forward (the Lambda instance, alpha = 1 in the
  * reference, cpaimd.C:2510-2516): L[x][y] = alpha * sum_k psi_x[k] F_y[k]
@@ -235,9 +258,12 @@ int oa_ortho_step(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
  * back-multiplies); S -> T is FP64 in every mode. */
 
 /* S[x][y] = alpha * sum_k psi_x[k] conj(psi_y[k])  (ZGEMM('C','N') with
- * right = left, ckPairCalculator.C:840-889), written exactly Hermitian. */
+ * right = left, ckPairCalculator.C:840-889), written exactly Hermitian.  If
+ * maxdev != NULL (device pointer) it receives Ortho::array_diag_max of S
+ * (ortho.h:282-313 with myabs = std::abs): the largest of |S_ij| off the
+ * diagonal and ||S_ii| - diag| on it. */
 int oa_zoverlap(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
-                double alpha, double *S, int prec);
+                double alpha, double *S, int prec, double diag, double *maxdev);
 
 /* Asymmetric complex forward: L[x][y] = alpha * sum_k psi_x[k] conj(F_y[k]). */
 int oa_zlambda(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
@@ -313,6 +339,38 @@ int oa_kx0_expand(oa_handle_t h, int n, int nfile, int nhalf,
                   const double *in, long ldin, double *out, long ldout);
 int oa_kx0_contract(oa_handle_t h, int n, int nfile, int nhalf,
                     const double *in, long ldin, double *out, long ldout);
+
+/* ---- caller-side transport (per-(state, plane) messages) ---------------
+ *
+ * The Charm++ driver moves the states to the PairCalculators and the
+ * results back as one message per (state, plane[, chunk]).  On one device
+ * the messages become copies into and out of the resident [n][ld] block.
+ * Message m covers rows row0[m] .. row0[m] + nrows[m] - 1 and doubles
+ * col0[m] .. col0[m] + ncols[m] - 1 of the block; its data are nrows[m] x
+ * ncols[m] contiguous doubles (row-major).  Messages must not overlap.
+ * *_on_host != 0 says every message buffer is host memory (staged through
+ * pinned memory of the handle), else every one is device memory.  The block
+ * is device memory.  Synchronous.  Returns the index of a bad argument:
+ * a message outside the block reports its row0 (3) or col0 (5). */
+
+/* Replaces MessageDataCollator::operator() (MessageDataCollator.h:163-247),
+ * which copies each paircalcInputMsg (pcMessages.h:117-153) to its sender's
+ * place in the collated buffer (:222). */
+int oa_collate_rows(oa_handle_t h, int count, const int *row0, const int *nrows,
+                    const long *col0, const int *ncols, const double *const *src,
+                    int src_on_host, double *block, long ld, int nblock_rows);
+
+/* The return leg: sendBWResultDirect (ckPairCalculator.C:2365-2452) sends
+ * one partialResultMsg per output state and plane, and the GSpace chare
+ * copies the first contribution and adds later ones with fastAdd
+ * (CP_State_GSpacePlane.C:3120-3126, 3197-3203; fastadd.C:34-51).  Message
+ * m's destination receives its slice of the block: copied when
+ * accumulate[m] == 0 (or accumulate == NULL), else added elementwise
+ * (dst += slice, one FP64 add per element as in fastAdd). */
+int oa_scatter_rows(oa_handle_t h, int count, const int *row0, const int *nrows,
+                    const long *col0, const int *ncols, double *const *dst,
+                    int dst_on_host, const int *accumulate, const double *block,
+                    long ld, int nblock_rows);
 
 /* ---- batched independent instances (grouped launches) -----------------
  *

@@ -28,23 +28,27 @@ OA_PREC_BF16 = 3
 OA_PREC_FP32 = 4
 OA_PREC_REUSE_DIGITS = 1 << 16
 OA_PREC_LEAN_BACKMUL = 1 << 12
+OA_PREC_ADAPTIVE = 1 << 13
 
 
 def prec_code(prec) -> int:
-    """'fp64' | 'int8x5' | 'int8x6l' | 'fp16x3' | 'bf16x4' | int -> C precision code.
-    A trailing 'l' selects the lean back-multiply (one digit level fewer)."""
+    """'fp64' | 'int8x5' | 'int8x5a' | 'int8x6l' | 'fp16x3' | 'bf16x4' | int -> C
+    precision code.  Suffix letters: 'l' selects the lean back-multiply (one
+    digit level fewer), 'a' the adaptive per-chunk slice counts (int8, 4-6
+    slices at most)."""
     if isinstance(prec, int):
         return prec
     p = prec.lower()
     if p == "fp64":
         return OA_PREC_FP64
-    m = re.fullmatch(r"(int8|fp16|bf16|fp32)(?:x(\d))?(l?)", p)
+    m = re.fullmatch(r"(int8|fp16|bf16|fp32)(?:x(\d))?(l?)(a?)", p)
     if not m:
         raise ValueError(f"unknown precision {prec!r}")
     base = {"int8": OA_PREC_INT8, "fp16": OA_PREC_FP16, "bf16": OA_PREC_BF16,
             "fp32": OA_PREC_FP32}[m.group(1)]
     slices = int(m.group(2) or 3)
-    return base | (slices << 8) | (OA_PREC_LEAN_BACKMUL if m.group(3) else 0)
+    return (base | (slices << 8) | (OA_PREC_LEAN_BACKMUL if m.group(3) else 0)
+            | (OA_PREC_ADAPTIVE if m.group(4) else 0))
 
 
 class OaError(RuntimeError):
@@ -103,6 +107,8 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_dgemm, _i, _vp, ctypes.c_char, ctypes.c_char, _i, _i, _i, _d, _vp, _i, _vp,
          _i, _d, _vp, _i)
     _sig(L.oa_overlap_sym, _i, _vp, _i, _i, _vp, _l, _i, _d, _i, _vp, _d, _vp)
+    _sig(L.oa_slice_counts, _i, _vp, _i, _i, _vp, _l, _i, _i, _vp, _i, ctypes.POINTER(_i),
+         ctypes.POINTER(_i))
     _sig(L.oa_lambda, _i, _vp, _i, _i, _vp, _l, _vp, _l, _d, _i, _vp)
     _sig(L.oa_s_to_t, _i, _vp, _i, _vp, _i, _d, _i, _vp, ctypes.POINTER(_i),
          ctypes.POINTER(_d))
@@ -110,7 +116,7 @@ def lib() -> ctypes.CDLL:
     _sig(L.oa_backmul, _i, _vp, _i, _i, _vp, _i, _vp, _l, _d, _d, _vp, _l, _i)
     _sig(L.oa_lambda_force, _i, _vp, _i, _i, _vp, _i, _vp, _l, _i, _d, _d, _d, _vp, _l, _i)
     _sig(L.oa_scale_kx0, _i, _vp, _i, _i, _vp, _l, _i, _d)
-    _sig(L.oa_zoverlap, _i, _vp, _i, _i, _vp, _l, _d, _vp, _i)
+    _sig(L.oa_zoverlap, _i, _vp, _i, _i, _vp, _l, _d, _vp, _i, _d, _vp)
     _sig(L.oa_state_read, _i, ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp,
          ctypes.POINTER(_i), ctypes.POINTER(_i), _vp)
     _sig(L.oa_state_write, _i, ctypes.c_char_p, _i, _i, _i, _vp, _vp, _vp, _vp, _vp)
@@ -127,6 +133,11 @@ def lib() -> ctypes.CDLL:
     _dp = ctypes.POINTER(ctypes.c_double)
     _sig(L.oa_zgemm, _i, _vp, ctypes.c_char, ctypes.c_char, _i, _i, _i, _dp, _vp, _i, _vp, _i,
          _dp, _vp, _i)
+    _pi, _pl = ctypes.POINTER(_i), ctypes.POINTER(_l)
+    _sig(L.oa_collate_rows, _i, _vp, _i, _pi, _pi, _pl, _pi, ctypes.POINTER(_vp), _i, _vp, _l,
+         _i)
+    _sig(L.oa_scatter_rows, _i, _vp, _i, _pi, _pi, _pl, _pi, ctypes.POINTER(_vp), _i, _pi,
+         _vp, _l, _i)
     _sig(L.oa_batch_create, _i, _vp, ctypes.POINTER(_vp))
     _sig(L.oa_batch_destroy, _i, _vp)
     _sig(L.oa_batch_plan, _i, _vp, _i, ctypes.POINTER(oa_instance), _i)

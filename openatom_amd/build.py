@@ -24,8 +24,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
          "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["oa_f64.hip", "oa_split.hip", "oa_batch.hip", "oa_complex.hip", "oa_kpoint.hip", "oa_statefile.hip",
-           "oa_api.hip"]
+SOURCES = ["oa_f64.hip", "oa_split.hip", "oa_batch.hip", "oa_complex.hip", "oa_kpoint.hip",
+           "oa_statefile.hip", "oa_transport.hip", "oa_api.hip"]
 
 
 def _compile(src: str) -> str:

@@ -92,6 +92,8 @@ size_t span_elems(int ld, int rows, int cols) {
 }  // namespace
 
 int oa_ws_require_into(oa_context *c, oa_workspace &w, size_t bytes) {
+  // every user of the main workspace may overwrite the digit area
+  if (&w == &c->ws) c->digits.valid = false;
   if (bytes <= w.bytes) return OA_SUCCESS;
   if (is_capturing(c->stream)) return OA_ERR_CAPACITY;  // no allocation under capture
   OA_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -160,6 +162,7 @@ int oa_destroy(oa_handle_t c) {
   if (c->ns.ptr) (void)hipFree(c->ns.ptr);
   if (c->report) (void)hipFree(c->report);
   if (c->host_report) (void)hipHostFree(c->host_report);
+  if (c->pinned.ptr) (void)hipHostFree(c->pinned.ptr);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (tl_default == c) tl_default = nullptr;
   delete c;
@@ -179,6 +182,25 @@ int oa_synchronize(oa_handle_t c) {
   DeviceGuard g(c->device);
   OA_HIP_CHECK(hipStreamSynchronize(c->stream));
   return OA_SUCCESS;
+}
+
+// Record that the workspace holds the digits of psi (both layouts after an
+// overlap, the back-multiply layout after a back-multiply; a reusing
+// back-multiply reads only the latter).
+static void tag_digits(oa_context *c, const double *psi, long ld, int n, int K, int prec) {
+  c->digits.valid = true;
+  c->digits.psi = psi;
+  c->digits.ld = ld;
+  c->digits.n = n;
+  c->digits.K = K;
+  c->digits.prec = prec & ~OA_PREC_REUSE_DIGITS;
+}
+
+static bool digits_match(const oa_context *c, const double *psi, long ld, int n, int K,
+                         int prec) {
+  const oa_digit_tag &t = c->digits;
+  return t.valid && t.psi == psi && t.ld == ld && t.n == n && t.K == K &&
+         t.prec == (prec & ~OA_PREC_REUSE_DIGITS);
 }
 
 static size_t ws_for(int n, int npw) {
@@ -351,10 +373,46 @@ int oa_overlap_sym(oa_handle_t c, int n, int npw, const double *psi, long ldpsi,
     return oa_launch_overlap_f64(c->stream, n, K, psi, ldpsi, 2 * nkx0, alpha, S, diag, maxdev,
                                  c->ws.ptr, c->ws.bytes);
   }
-  int st = oa_ws_require(c, oa_split_ws_bytes(prec, n, K));
+  const size_t need = oa_split_ws_bytes(prec, n, K);
+  if (need == 0) return OA_ERR_UNSUPPORTED;
+  int st = oa_ws_require(c, need);
   if (st) return st;
-  return oa_launch_overlap_split(c->stream, prec, n, K, psi, ldpsi, 2 * nkx0, alpha, S, diag,
-                                 maxdev, c->ws.ptr, c->ws.bytes);
+  st = oa_launch_overlap_split(c->stream, prec, n, K, psi, ldpsi, 2 * nkx0, alpha, S, diag,
+                               maxdev, c->ws.ptr, c->ws.bytes);
+  if (st == OA_SUCCESS) tag_digits(c, psi, ldpsi, n, K, prec);
+  return st;
+}
+
+int oa_slice_counts(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, int nkx0,
+                    int prec, int *counts, int capacity, int *nchunks, int *kchunk) {
+  if (!c) return OA_ERR_NO_DEVICE;
+  if (n < 1) return 2;
+  if (npw < 1) return 3;
+  if (!psi) return 4;
+  if (ldpsi < 2L * npw) return 5;
+  if (nkx0 < 0 || nkx0 > npw) return 6;
+  if (capacity < 0 || (capacity > 0 && !counts)) return 8;
+  DeviceGuard g(c->device);
+  const int K = 2 * npw;
+  const size_t need = oa_split_ws_bytes(prec, n, K);
+  if (need == 0 || !(prec & OA_PREC_ADAPTIVE)) return OA_ERR_UNSUPPORTED;
+  int st = oa_ws_require(c, need);
+  if (st) return st;
+  const int *dcounts = nullptr;
+  int nz = 0, kc = 0;
+  st = oa_launch_slice_counts(c->stream, prec, n, K, psi, ldpsi, 2 * nkx0, c->ws.ptr,
+                              c->ws.bytes, &dcounts, &nz, &kc);
+  if (st) return st;
+  if (nchunks) *nchunks = nz;
+  if (kchunk) *kchunk = kc;
+  if (nz > capacity) {
+    OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return OA_ERR_CAPACITY;
+  }
+  OA_HIP_CHECK(hipMemcpyAsync(counts, dcounts, (size_t)nz * sizeof(int), hipMemcpyDeviceToHost,
+                              c->stream));
+  OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return OA_SUCCESS;
 }
 
 int oa_lambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, const double *F,
@@ -446,10 +504,20 @@ static int backmul_cols(oa_context *c, int n, int npw, const double *M, int tran
   DeviceGuard g(c->device);
   const int K = 2 * npw;
   if ((prec & 0xff) != OA_PREC_FP64) {
-    int st = oa_ws_require(c, oa_split_ws_bytes(prec, n, K));
+    // reuse is a hint: honoured only while the workspace still holds this
+    // psi's digits at this precision
+    const int chk = oa_split_check(prec, n, 128);
+    if (chk) return chk;
+    if ((prec & OA_PREC_REUSE_DIGITS) && !digits_match(c, psi, ldpsi, n, K, prec))
+      prec &= ~OA_PREC_REUSE_DIGITS;
+    const size_t need = oa_split_ws_bytes(prec, n, K);
+    if (need == 0) return OA_ERR_UNSUPPORTED;
+    int st = oa_ws_require(c, need);
     if (st) return st;
-    return oa_launch_backmul_split(c->stream, prec, n, K, M, transM, psi, ldpsi, 0, alpha, beta,
-                                   out, ldout, c->ws.ptr, c->ws.bytes, cw, alpha_w);
+    st = oa_launch_backmul_split(c->stream, prec, n, K, M, transM, psi, ldpsi, 0, alpha, beta,
+                                 out, ldout, c->ws.ptr, c->ws.bytes, cw, alpha_w);
+    if (st == OA_SUCCESS) tag_digits(c, psi, ldpsi, n, K, prec);
+    return st;
   }
   oa_gemm_problem p;
   p.M = n;
@@ -527,6 +595,10 @@ int oa_ortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, 
   if (ldout < 2L * npw) return 14;
   if (n == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
+  if ((prec & 0xff) != OA_PREC_FP64) {
+    const int chk = oa_split_check(prec, n, 128);
+    if (chk) return chk;
+  }
   int st = oa_reserve_prec(c, n, npw, prec);
   if (st) return st;
   st = oa_overlap_sym(c, n, npw, psi, ldpsi, nkx0, 2.0, prec, S, 0.0, nullptr);

@@ -8,10 +8,11 @@
 // never exchange data on this path, so they batch without communication.
 //
 // Sequence of one step (split precision):
-//   fill counters -> chunk max -> slice -> overlap digit products (one
-//   launch per digit tile in use) -> reduce -> S->T init -> maxit x (GEMM
-//   C = 3I - AB, GEMM pair B' = BC/2, A' = CA/2, residual, update) -> finish
-//   -> slice T -> back-multiply digit products.
+//   fill counters -> chunk max -> (adaptive: slice counts) -> slice ->
+//   overlap digit products (one launch per digit tile in use) -> reduce ->
+//   S->T init -> maxit x (GEMM C = 3I - AB, GEMM pair B' = BC/2,
+//   A' = CA/2, residual, update) -> finish -> slice T -> back-multiply
+//   digit products.
 // FP64: the overlap and the back-multiply are per-instance FP64-MFMA launches
 // inside the same sequence; S->T is grouped as above.
 // Every launch reads device tables built at plan time; the sequence performs
@@ -222,6 +223,15 @@ int plan(oa_batch *b, int count, const oa_instance *in, int prec) {
   if (!dg.ok) return OA_ERR_NO_DEVICE;
   const bool split = (prec & 0xff) != OA_PREC_FP64;
   if (split && oa_split_ws_bytes(prec, 64, 64) == 0) return OA_ERR_UNSUPPORTED;
+  // every instance must fit the digit kernels' exact int32 sums (checked
+  // before the batch's state changes)
+  if (split)
+    for (int i = 0; i < count; i++) {
+      auto t = b->table.find(cdivi(in[i].n, 32));
+      const int tile = (t != b->table.end()) ? t->second.first : default_tile(in[i].n);
+      const int chk = oa_split_check(prec, in[i].n, tile);
+      if (chk) return chk;
+    }
   OA_HIP_CHECK(hipStreamSynchronize(b->gstream));
   OA_HIP_CHECK(hipStreamSynchronize(c->stream));
   drop_graph(b);
@@ -314,6 +324,11 @@ int plan(oa_batch *b, int count, const oa_instance *in, int prec) {
         blob, b->seq, LK_SPLIT, OA_SP_CHUNKMAX, 128, b->ip,
         [&](const InstPlan &, int i) { return sa[i].cm; },
         [&](const InstPlan &, int i) { return sa[i].items_cm; }, all, fix);
+    if (prec & OA_PREC_ADAPTIVE)
+      add_group<oa_count_args>(
+          blob, b->seq, LK_SPLIT, OA_SP_COUNT, 128, b->ip,
+          [&](const InstPlan &, int i) { return sa[i].cnt; },
+          [&](const InstPlan &, int) { return 1; }, all, fix);
     add_group<oa_slice_args>(
         blob, b->seq, LK_SPLIT, OA_SP_SLICE, 128, b->ip,
         [&](const InstPlan &, int i) { return sa[i].sl; },

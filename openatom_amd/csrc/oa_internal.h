@@ -24,6 +24,16 @@ struct oa_workspace {
   size_t bytes = 0;
 };
 
+// What the digit slices in a handle's workspace were made from: the last
+// split slicing on the handle.  Any other use of the workspace clears it, so a
+// back-multiply asked to reuse digits re-slices unless they are still there.
+struct oa_digit_tag {
+  bool valid = false;
+  const double *psi = nullptr;
+  long ld = 0;
+  int n = 0, K = 0, prec = 0;
+};
+
 struct oa_context {
   int device = 0;
   hipStream_t own_stream = nullptr;
@@ -32,6 +42,8 @@ struct oa_context {
   oa_workspace ns;          // S->T ping-pong buffers (kept apart so digits survive)
   oa_ns_report *report = nullptr;  // device report for synchronous S->T
   oa_ns_report *host_report = nullptr;  // pinned mirror
+  oa_digit_tag digits;      // contents of ws's digit area
+  oa_workspace pinned;      // pinned host staging of the transport calls
 };
 
 int oa_ws_require(oa_context *c, size_t bytes);
@@ -180,6 +192,13 @@ struct oa_slice_args {
   void *DS, *DB;
   int np, Kp, want_ds, vec;
   int want_db, pad;
+  const int *slices;  // per chunk digit count (adaptive), null: all S
+};
+// Per chunk slice counts of the adaptive split (one workgroup per instance).
+struct oa_count_args {
+  const unsigned long long *maxB;
+  int *slices;
+  int nz, K, kchunk, smax;
 };
 struct oa_digit_args {
   const void *X;  // k-blocked digits, rows m
@@ -204,6 +223,7 @@ struct oa_digit_args {
   int kchunk;
   int cw;                            // MODE 1: columns < cw use alpha_w
   double alpha_w;
+  const int *slices;                 // adaptive: digits per chunk (null: S)
 };
 struct oa_sreduce_args {
   const double *part;
@@ -228,18 +248,25 @@ struct oa_rows_args {
 // Split-precision layout of one instance (digit workspace offsets).
 struct oa_split_plan {
   int dt, S, es, tile;
+  int adaptive;     // per chunk slice counts (OA_PREC_ADAPTIVE)
   int np, Kp;       // padded rows / columns (multiples of the tile)
   int kchunk, nz;   // psi chunks: scale blocks and split-K units
   int ntiles;       // upper-triangle tiles of S
-  size_t off_ds, off_db, off_dt, off_maxS, off_maxB, off_fexp, off_diag, off_part, off_md;
+  size_t off_ds, off_db, off_dt, off_maxS, off_maxB, off_fexp, off_diag, off_part, off_md,
+      off_sl;
   size_t total;
 };
 oa_split_plan oa_split_plan_for(int prec, int n, int K, int tile);
+// OA_SUCCESS when a split back-multiply of n states fits the digit kernels'
+// exact int32 sums (OA_ERR_CAPACITY otherwise), OA_ERR_UNSUPPORTED for an
+// unknown split precision.
+int oa_split_check(int prec, int n, int tile);
 // Fill the per-phase argument structs of one instance whose digit workspace
 // starts at `ws` (layout `pl`).  maxS/maxB may point outside ws (batched
 // counters); null selects the plan's own slots.
 struct oa_split_inst_args {
   oa_chunkmax_args cm;
+  oa_count_args cnt;
   oa_slice_args sl;
   oa_digit_args d0;
   oa_sreduce_args rd;
@@ -253,11 +280,17 @@ void oa_split_inst_fill(const oa_split_plan &pl, int prec, int n, int K, const d
                         unsigned long long *maxS, unsigned long long *maxB,
                         oa_split_inst_args *a);
 enum oa_split_phase {
-  OA_SP_CHUNKMAX = 0, OA_SP_SLICE, OA_SP_DIGIT0, OA_SP_REDUCE, OA_SP_ROWS, OA_SP_DIGIT1
+  OA_SP_CHUNKMAX = 0, OA_SP_SLICE, OA_SP_DIGIT0, OA_SP_REDUCE, OA_SP_ROWS, OA_SP_DIGIT1,
+  OA_SP_COUNT
 };
 // One grouped launch of a split phase for instances sharing `prec` and `tile`.
 int oa_split_grouped(hipStream_t s, int prec, int tile, int phase, const oa_group &g);
 int oa_fill_u64(hipStream_t s, unsigned long long *p, long n);
+// Chunk maxima and adaptive slice counts of psi (a split plan of `prec` in
+// ws); *counts receives the device array of nz counts.
+int oa_launch_slice_counts(hipStream_t s, int prec, int n, int K, const double *psi, long ld,
+                           int kx0_end, void *ws, size_t ws_bytes, const int **counts,
+                           int *nz, int *kchunk);
 
 // FP64 grouped kernels: GEMM (row-major NN, per-problem skip flag) and the
 // S->T bookkeeping kernels.

@@ -71,14 +71,27 @@ __global__ void zmerge_kernel(long nn, const double *__restrict__ re,
 // Hermitian S from the symmetric real part Sr and the imaginary part Si of
 // the upper triangle: S[x][y] = Sr[x][y] + i Si[x][y] (x < y), its conjugate
 // mirror below, a real diagonal.
+// With md != null it also takes Ortho::array_diag_max of the merged S
+// (ortho.h:282-313 with myabs = std::abs): |S_ij| off the diagonal and
+// ||S_ii| - diag| on it, maximised through atomicMax on the IEEE bits of
+// non-negative doubles (md cleared beforehand).
 __global__ void zherm_kernel(int n, const double *__restrict__ Sr, const double *__restrict__ Si,
-                             double *__restrict__ S) {
+                             double *__restrict__ S, double diag, unsigned long long *md) {
   const long nn = (long)n * n;
+  double mx = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
        i += (long)gridDim.x * blockDim.x) {
     const long x = i / n, y = i % n;
-    S[2 * i] = Sr[i];
-    S[2 * i + 1] = (x < y) ? Si[i] : ((x > y) ? -Si[y * n + x] : 0.0);
+    const double re = Sr[i], im = (x < y) ? Si[i] : ((x > y) ? -Si[y * n + x] : 0.0);
+    S[2 * i] = re;
+    S[2 * i + 1] = im;
+    const double a = hypot(re, im);
+    mx = fmax(mx, (x == y) ? fabs(a - diag) : a);
+  }
+  if (md) {
+    // wave maximum, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(md, (unsigned long long)__double_as_longlong(mx));
   }
 }
 
@@ -148,17 +161,20 @@ int zmul_planes(hipStream_t s, int n, double alpha, const double *Ar, const doub
   return gemm2(s, n, Ai, Bi, Cr, -alpha, 1.0, 0.0, Ai, Br, Ci, alpha, 1.0);
 }
 
-// Workspace of the complex forward and backward: J psi ([n][K]), two n x n
-// planes, then (forward only) the FP64 kernels' own workspace; every part
-// starts on a 256-byte boundary.
-size_t al256(size_t b) { return (b + 255) / 256 * 256; }
-size_t off_planes(int n, int K) { return al256((size_t)n * K * sizeof(double)); }
-size_t plane_bytes(int n) { return al256((size_t)n * n * sizeof(double)); }
-size_t off_kws(int n, int K) { return off_planes(n, K) + 2 * plane_bytes(n); }
-
+// Workspace of the complex forward and backward: J psi ([n][K]) where the
+// products read it, two n x n planes, then the kernels' own workspace; every
+// part starts on a 256-byte boundary.  J psi is formed in FP64 by the FP64
+// products and by the general split forward (which slices it); the split
+// Hermitian forward and the split back-multiply make J psi's digits from
+// psi's and need no J region, so a split step lays out no J.
 bool is_split(int prec) { return (prec & 0xff) != OA_PREC_FP64; }
+size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+bool needs_j(bool sym_or_back, int prec) { return !(sym_or_back && is_split(prec)); }
+size_t off_planes(int n, int K, bool jn) { return jn ? al256((size_t)n * K * sizeof(double)) : 0; }
+size_t plane_bytes(int n) { return al256((size_t)n * n * sizeof(double)); }
+size_t off_kws(int n, int K, bool jn) { return off_planes(n, K, jn) + 2 * plane_bytes(n); }
 
-size_t zoverlap_ws_bytes(int n, int K, int prec) {
+size_t zoverlap_ws_bytes(int n, int K, int prec, bool sym) {
   size_t kws = oa_overlap_f64_ws_bytes(n, K);
   if (oa_ntsplit_f64_ws_bytes(n, n, K) > kws) kws = oa_ntsplit_f64_ws_bytes(n, n, K);
   if (is_split(prec)) {
@@ -166,11 +182,11 @@ size_t zoverlap_ws_bytes(int n, int K, int prec) {
     if (oa_split_lambda_ws_bytes(prec, n, K) > kws) kws = oa_split_lambda_ws_bytes(prec, n, K);
     if (oa_split_zimag_ws_bytes(prec, n, K) > kws) kws = oa_split_zimag_ws_bytes(prec, n, K);
   }
-  return off_kws(n, K) + kws;
+  return off_kws(n, K, needs_j(sym, prec)) + kws;
 }
 
 size_t zbackmul_ws_bytes(int n, int K, int prec) {
-  return off_kws(n, K) + (is_split(prec) ? oa_split_ws_bytes(prec, n, K) : 0);
+  return off_kws(n, K, needs_j(true, prec)) + (is_split(prec) ? oa_split_ws_bytes(prec, n, K) : 0);
 }
 
 // Hermitian (sym) or general (!sym) product L[x][y] = alpha sum_k psi_x[k]
@@ -178,13 +194,14 @@ size_t zbackmul_ws_bytes(int n, int K, int prec) {
 // real and imaginary planes and the kernel workspace follow it in ws.
 int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld,
              const double *phi, long ldphi, double alpha, double *L, char *ws, size_t ws_bytes,
-             int prec) {
+             int prec, double diag = 0.0, double *maxdev = nullptr) {
   const int K = 2 * npw;
-  double *J = reinterpret_cast<double *>(ws);
-  double *Lr = reinterpret_cast<double *>(ws + off_planes(n, K));
-  double *Li = reinterpret_cast<double *>(ws + off_planes(n, K) + plane_bytes(n));
-  char *kws = ws + off_kws(n, K);
-  const size_t kbytes = ws_bytes - off_kws(n, K);
+  const bool jn = needs_j(sym, prec);
+  double *J = jn ? reinterpret_cast<double *>(ws) : nullptr;
+  double *Lr = reinterpret_cast<double *>(ws + off_planes(n, K, jn));
+  double *Li = reinterpret_cast<double *>(ws + off_planes(n, K, jn) + plane_bytes(n));
+  char *kws = ws + off_kws(n, K, jn);
+  const size_t kbytes = ws_bytes - off_kws(n, K, jn);
   // the split Hermitian forward takes J psi's digits from psi's; every other
   // form multiplies J phi in FP64 or slices it
   if (!(sym && is_split(prec)))
@@ -217,9 +234,11 @@ int zforward(hipStream_t s, bool sym, int n, int npw, const double *psi, long ld
     }
   }
   if (st) return st;
-  if (sym)
-    OA_KLAUNCH(zherm_kernel, (long)n * n, s, n, Lr, Li, L);
-  else
+  if (sym) {
+    auto *md = reinterpret_cast<unsigned long long *>(maxdev);
+    if (md && oa_fill_u64(s, md, 1)) return OA_ERR_HIP;
+    OA_KLAUNCH(zherm_kernel, (long)n * n, s, n, Lr, Li, L, diag, md);
+  } else
     OA_KLAUNCH(zmerge_kernel, (long)n * n, s, (long)n * n, Lr, Li, L);
   return OA_SUCCESS;
 }
@@ -231,17 +250,18 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
               long ld, double alpha, double beta, double *out, long ldo, char *ws,
               size_t ws_bytes, bool have_j, int prec, bool reuse_psi = false) {
   const int K = 2 * npw;
-  double *J = reinterpret_cast<double *>(ws);
-  double *Mr = reinterpret_cast<double *>(ws + off_planes(n, K));
-  double *Mi = reinterpret_cast<double *>(ws + off_planes(n, K) + plane_bytes(n));
+  const bool jn = needs_j(true, prec);
+  double *J = jn ? reinterpret_cast<double *>(ws) : nullptr;
+  double *Mr = reinterpret_cast<double *>(ws + off_planes(n, K, jn));
+  double *Mi = reinterpret_cast<double *>(ws + off_planes(n, K, jn) + plane_bytes(n));
   // split modes make J psi's digits from psi's and never read J psi
   if (!have_j && !is_split(prec))
     OA_KLAUNCH(jswap_kernel, (long)n * npw, s, n, npw, psi, ld, J, (long)K);
   OA_KLAUNCH(zplanes_kernel, (long)n * n, s, (long)n * n, M, 1.0, Mr, Mi);
   // Re(M^H) = Mr^T and Im(M^H) = -Mi^T
   if (is_split(prec)) {
-    char *kws = ws + off_kws(n, K);
-    const size_t kbytes = ws_bytes - off_kws(n, K);
+    char *kws = ws + off_kws(n, K, jn);
+    const size_t kbytes = ws_bytes - off_kws(n, K, jn);
     const int p = prec & ~OA_PREC_REUSE_DIGITS;
     int st = oa_launch_backmul_split(s, reuse_psi ? (p | OA_PREC_REUSE_DIGITS) : p, n, K, Mr,
                                      transM, psi, ld, 0, alpha, beta, out, ldo, kws, kbytes);
@@ -250,7 +270,9 @@ int zbackward(hipStream_t s, int n, int npw, const double *M, int transM, const 
     // instead of slicing J psi; the second product reuses them
     st = oa_launch_jswap_db(s, p, n, K, kws, kbytes);
     if (st) return st;
-    return oa_launch_backmul_split(s, p | OA_PREC_REUSE_DIGITS, n, K, Mi, transM, J, K, 0,
+    // psi's pointer and pitch stand in for J psi's: with the digits reused
+    // the product reads neither
+    return oa_launch_backmul_split(s, p | OA_PREC_REUSE_DIGITS, n, K, Mi, transM, psi, ld, 0,
                                    transM ? -alpha : alpha, 1.0, out, ldo, kws, kbytes);
   }
   oa_gemm_problem p{n, K, n, Mr, n, psi, ld, out, ldo, alpha, beta, 0.0};
@@ -325,8 +347,9 @@ size_t zs_to_t_ws_bytes(int n) {
 extern "C" {
 
 int oa_zoverlap(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, double alpha,
-                double *S, int prec) {
+                double *S, int prec, double diag, double *maxdev) {
   if (!c) return OA_ERR_NO_DEVICE;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   if (n < 0) return 2;
   if (npw < 0) return 3;
   if (n > 0 && npw > 0 && !psi) return 4;
@@ -336,17 +359,24 @@ int oa_zoverlap(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, do
   DeviceGuard g(c->device);
   if (npw == 0) {
     OA_HIP_CHECK(hipMemsetAsync(S, 0, 2 * (size_t)n * n * sizeof(double), c->stream));
+    if (maxdev) {
+      // S = 0: array_diag_max is |0 - diag| on the diagonal
+      const double d = fabs(diag);
+      OA_HIP_CHECK(hipMemcpyAsync(maxdev, &d, sizeof(double), hipMemcpyHostToDevice, c->stream));
+      OA_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
     return OA_SUCCESS;
   }
-  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw, prec));
+  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw, prec, true));
   if (st) return st;
   return zforward(c->stream, true, n, npw, psi, ldpsi, psi, ldpsi, alpha, S, (char *)c->ws.ptr,
-                  c->ws.bytes, prec);
+                  c->ws.bytes, prec, diag, maxdev);
 }
 
 int oa_zlambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, const double *F,
                long ldF, double alpha, double *L, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   if (n < 0) return 2;
   if (npw < 0) return 3;
   if (n > 0 && npw > 0 && !psi) return 4;
@@ -360,7 +390,7 @@ int oa_zlambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, con
     OA_HIP_CHECK(hipMemsetAsync(L, 0, 2 * (size_t)n * n * sizeof(double), c->stream));
     return OA_SUCCESS;
   }
-  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw, prec));
+  const int st = oa_ws_require(c, zoverlap_ws_bytes(n, 2 * npw, prec, false));
   if (st) return st;
   return zforward(c->stream, false, n, npw, psi, ldpsi, F, ldF, alpha, L, (char *)c->ws.ptr,
                   c->ws.bytes, prec);
@@ -384,6 +414,7 @@ int oa_zs_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int
 int oa_zbackmul(oa_handle_t c, int n, int npw, const double *M, int transM, const double *psi,
                 long ldpsi, double alpha, double beta, double *out, long ldout, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   if (n < 0) return 2;
   if (npw < 0) return 3;
   if (n > 0 && !M) return 4;
@@ -403,6 +434,7 @@ int oa_zortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi,
                    double tol, int maxit, int modulus, double *S, double *T, double *psi_out,
                    long ldout, int *iters, double *err, int prec) {
   if (!c) return OA_ERR_NO_DEVICE;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   if (n < 0) return 2;
   if (npw < 1) return 3;
   if (n > 0 && !psi) return 4;
@@ -414,7 +446,7 @@ int oa_zortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi,
   if (n == 0) return OA_SUCCESS;
   DeviceGuard g(c->device);
   const int K = 2 * npw;
-  size_t need = zoverlap_ws_bytes(n, K, prec);
+  size_t need = zoverlap_ws_bytes(n, K, prec, true);
   if (zbackmul_ws_bytes(n, K, prec) > need) need = zbackmul_ws_bytes(n, K, prec);
   int st = oa_ws_require(c, need);
   if (!st) st = oa_ws_require_into(c, c->ns, zs_to_t_ws_bytes(n));

@@ -50,6 +50,8 @@
 //                 mirror, max|S - d I|.
 #include <math.h>
 
+#include <type_traits>
+
 #include "oa_internal.h"
 
 namespace {
@@ -63,6 +65,8 @@ typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 constexpr int DBK = 32;   // chunk granularity (elements); a stage holds 32 bytes per row
 constexpr int SMAX = 6;
 constexpr int SLT = 64;   // slicing tile (rows x columns)
+constexpr int ADA_SMIN = 2;      // fewest slices an adaptive chunk keeps
+constexpr int ADA_SMIN_MAX = 4;  // adaptive mode needs at least this many slices at most
 constexpr long KCHUNK_TARGET = 8192;  // columns per scale block / split-K chunk
 
 __host__ __device__ inline int cdv(int a, int b) { return (a + b - 1) / b; }
@@ -308,6 +312,76 @@ __global__ __launch_bounds__(256) void chunk_max_grouped(const oa_chunkmax_args 
   });
 }
 
+// ------------------------------------------------------------ slice counts
+// Adaptive split: the digit count s_z of every chunk from the chunk maxima
+// alone (one workgroup per instance).  The truncation of a chunk's digits
+// contributes at most c_s K_z 4^(e_z) 254^-(s_z - 1) to every element of S
+// (kept levels p + q < s_z plus the remainders of s_z digits; c_s grows
+// slowly with s), and the same weight W_z = K_z 4^(e_z) prices the error the
+// chunk's columns of T psi add to the new states' overlap.  With
+// W = sum_z W_z, chunk z drops d slices when W_z 254^d <= W / nz, so the
+// dropped chunks add at most W 254^-(S-1) c_S: the bound stays within twice
+// that of S uniform slices.  Weights are taken relative to the largest
+// exponent, and both sums run in a fixed order, so every launch form gives
+// the same counts.
+__device__ __forceinline__ double chunk_weight(const oa_count_args &a, int z, int eref) {
+  const double m = __longlong_as_double((long long)a.maxB[z]);
+  if (!(m > 0.0)) return 0.0;
+  const long k0 = (long)z * a.kchunk, k1 = min((long)a.K, k0 + a.kchunk);
+  const double kz = (double)max(0L, k1 - k0);
+  return ldexp(kz, 2 * (chunk_exp(m) - eref));
+}
+
+__device__ __forceinline__ void slice_count_body(const oa_count_args &a) {
+  __shared__ double red[256];
+  __shared__ int ered[256];
+  const int t = threadIdx.x;
+  int em = -2147483647;
+  for (int z = t; z < a.nz; z += 256) {
+    const double m = __longlong_as_double((long long)a.maxB[z]);
+    if (m > 0.0) em = max(em, chunk_exp(m));
+  }
+  ered[t] = em;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) ered[t] = max(ered[t], ered[t + o]);
+    __syncthreads();
+  }
+  const int eref = ered[0];
+  double w = 0.0;
+  for (int z = t; z < a.nz; z += 256) w += chunk_weight(a, z, eref);
+  red[t] = w;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) red[t] += red[t + o];
+    __syncthreads();
+  }
+  const double budget = red[0] / (double)a.nz;
+  for (int z = t; z < a.nz; z += 256) {
+    double wz = chunk_weight(a, z, eref);
+    int s = a.smax;
+    while (s > ADA_SMIN && wz * RADIX <= budget) {
+      wz *= RADIX;
+      s--;
+    }
+    a.slices[z] = s;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void slice_count_kernel(oa_count_args a) {
+  slice_count_body(a);
+}
+
+__global__ __launch_bounds__(256) void slice_count_grouped(const oa_count_args *__restrict__ t,
+                                                           const int *__restrict__ prefix,
+                                                           int count) {
+  for_items(prefix[count], [&](int item) {
+    const int i = group_find(prefix, count, item);
+    slice_count_body(t[i]);
+  });
+}
+
 // ------------------------------------------------------------ slicing
 // k-blocked element offset (elements) of (row r, contraction c) in one slice.
 template <int DT>
@@ -339,6 +413,8 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
   const int kc = k0 + 2 * cp;  // first of this thread's two columns
   const int z = min(kc, a.K - 1 > 0 ? a.K - 1 : 0) / a.kchunk;
   const double sS = chunk_scale(a.maxS, z), sB = chunk_scale(a.maxB, z);
+  // digits kept for this chunk (the item's 64 columns lie in one chunk)
+  const int sz = a.slices ? a.slices[z] : S;
   const size_t sds = (size_t)a.np * a.Kp, sdb = (size_t)a.Kp * a.np;
   // all eight 16-byte loads first (eight in flight per thread)
   double xq[8][2];
@@ -380,6 +456,7 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
     if (a.want_ds) {
 #pragma unroll
       for (int p = 0; p < S; p++) {
+        if (p >= sz) break;
         E *dst = DS + p * sds + kb_off<DT>(i, kc, a.np);
         if constexpr (ES == 1) {
           const unsigned short u = (unsigned short)(((unsigned)(int)dS[0][p] & 0xffu) |
@@ -401,7 +478,7 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
   const int kk = t / 4, ib = (t % 4) * 16;
 #pragma unroll
   for (int p = 0; p < S; p++) {
-    if (!a.want_db) break;
+    if (!a.want_db || p >= sz) break;
     E *dst = DB + p * sdb;
     const i4v src = *reinterpret_cast<const i4v *>(&lt[p][kk * PITCH + ib]);
     if constexpr (ES == 1) {
@@ -601,19 +678,24 @@ struct DigitCfg {
 };
 
 // C[m][n] = sum over the contraction of level-grouped digit products, for one
-// output tile (item `logical`).
-//   MODE 0: symmetric split-K partial tiles of S; S slices, levels < S.
+// output tile (bi, bj) and, in MODES 0 and 2, one chunk z.
+//   MODE 0: symmetric split-K partial tiles of S; NLR slices, levels < NLR.
 //   MODE 1: back-multiply with T = D + X: out[m][n] = alpha (D_m psi~[m][n]
-//           + 2^(f_m + e(n) - 14) C) + beta out, where C uses NLB slices of
-//           X and of psi (levels < NLB) and psi~ is psi rebuilt from all S
-//           digits of DB.
-// The X and Y panels of NBUF stages rotate through LDS, filled by
-// direct-to-LDS loads (global_load_lds_dwordx4) NBUF-1 stages ahead.
-template <int DT, int S, int MODE, int NLB, int TILE>
-__device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, char *smem) {
+//           + 2^(f_m + e(n) - 14) C) + beta out, where C uses NLR slices of
+//           X and of psi (levels < NLR) and psi~ is psi rebuilt from the
+//           first SD digits of DB.
+//   MODE 2: all-tiles split-K partial tiles (Lambda).
+// NLR is NL (S or NLB) for a uniform split; an adaptive split selects it per
+// item from the chunk's slice count.  The X and Y panels of NBUF stages
+// (slots sized for NL slices) rotate through LDS, filled by direct-to-LDS
+// loads (global_load_lds_dwordx4) NBUF-1 stages ahead.
+template <int DT, int S, int MODE, int NLB, int TILE, int NLR>
+__device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int bj, int z,
+                                           int tile, int SD, char *smem) {
   typedef typename DTraits<DT>::acc_t ACC;
   typedef DigitCfg<DT, S, MODE, NLB, TILE> C;
-  constexpr int NL = C::NL;
+  constexpr int NL = NLR;
+  static_assert(NLR >= 1 && NLR <= C::NL, "levels within the ring's slots");
   constexpr int ES = DTraits<DT>::ES;
   constexpr int SK = 32 / ES;             // contraction elements per stage
   constexpr int PANEL = C::PANEL;
@@ -633,22 +715,6 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   // FP64 sum per output
   constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 128 / SK;
 
-  int bi, bj, z = 0, tile = logical;
-  if (MODE == 0) {
-    tile = logical % a.ntiles;
-    z = logical / a.ntiles;
-    upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
-  } else if (MODE == 2) {
-    const int T = cdv(a.M, TILE);
-    tile = logical % a.ntiles;
-    z = logical / a.ntiles;
-    bi = tile % T;
-    bj = tile / T;
-  } else {
-    const int tm = cdv(a.M, TILE);
-    bi = logical % tm;
-    bj = logical / tm;
-  }
   const int m0 = bi * TILE, n0 = bj * TILE;
   const int ka = (MODE != 1) ? z * a.kc : 0;
   const int kb = (MODE != 1) ? min(a.kfull, ka + a.kc) : a.kfull;
@@ -798,6 +864,7 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
       const int b0 = 8 * g + 4 * (lane >> 5);
 #pragma unroll
       for (int p = S - 1; p >= 0; p--) {
+        if (p >= SD) continue;
         double q4[4];
         if (DIAG_LDS && p < NL) {
           const char *yb = smem + ((nst - ND + xr / SK) % NBUF) * STAGE + NL * PANEL + p * PANEL;
@@ -840,6 +907,76 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
   }
   // the next item's first loads overwrite the ring (stores may stay in flight)
   ring_barrier();
+}
+
+// Calls f(integral_constant<int, V>) with V = v clamped into [LO, HI].
+template <int LO, int HI, class F>
+__device__ __forceinline__ void level_switch(int v, F &&f) {
+  if constexpr (LO < HI) {
+    if (v <= LO) f(std::integral_constant<int, LO>{});
+    else level_switch<LO + 1, HI>(v, f);
+  } else {
+    f(std::integral_constant<int, HI>{});
+  }
+}
+
+// Item decoding.  An adaptive split spreads the chunks over the XCDs: the
+// dispatcher gives XCD x the contiguous item range x*per .. (x+1)*per - 1,
+// which covers consecutive chunks; with nz = 8c the chunk order is permuted
+// so that XCD x takes chunks x, x + 8, ..., and every XCD gets a like mix of
+// heavy (leading, more slices) and light chunks.  Tiles of one chunk stay on
+// one XCD (their operand panels share its L2).
+template <int DT, int S, int MODE, int NLB, int TILE>
+__device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, char *smem) {
+  constexpr int NL = DigitCfg<DT, S, MODE, NLB, TILE>::NL;
+  constexpr bool ADA_OK = (DT == DT_I8) && (MODE != 2) && (S >= ADA_SMIN_MAX);
+  const bool ada = ADA_OK && a.slices != nullptr;
+  int bi, bj, z = 0, tile = logical;
+  if (MODE == 0) {
+    tile = logical % a.ntiles;
+    z = logical / a.ntiles;
+    if (ada && (a.nz & 7) == 0) {
+      const int c = a.nz >> 3;
+      z = (z % c) * 8 + z / c;
+    }
+    upper_tile_d(tile, cdv(a.M, TILE), bi, bj);
+  } else if (MODE == 2) {
+    const int T = cdv(a.M, TILE);
+    tile = logical % a.ntiles;
+    z = logical / a.ntiles;
+    bi = tile % T;
+    bj = tile / T;
+  } else {
+    const int tm = cdv(a.M, TILE);
+    bi = logical % tm;
+    bj = logical / tm;
+    if (ada) {
+      // chunk of the output columns (kchunk is a multiple of TILE)
+      const int cpt = a.kchunk / TILE;
+      if ((a.nz & 7) == 0 && cdv(a.N, TILE) == a.nz * cpt) {
+        const int c = a.nz >> 3, zl = bj / cpt;
+        bj = ((zl % c) * 8 + zl / c) * cpt + bj % cpt;
+      }
+      z = bj / cpt;
+    }
+  }
+  if constexpr (ADA_OK) {
+    if (ada) {
+      const int sz = a.slices[z];
+      if constexpr (MODE == 0) {
+        level_switch<ADA_SMIN, NL>(sz, [&](auto c) {
+          digit_core<DT, S, MODE, NLB, TILE, decltype(c)::value>(a, bi, bj, z, tile, S, smem);
+        });
+      } else {
+        constexpr int LO = (NLB - (S - ADA_SMIN) > 1) ? NLB - (S - ADA_SMIN) : 1;
+        level_switch<LO, NLB>(NLB - (S - sz), [&](auto c) {
+          digit_core<DT, S, MODE, NLB, TILE, decltype(c)::value>(a, bi, bj, z, tile, sz, smem);
+        });
+      }
+      return;
+    }
+  }
+  digit_core<DT, S, MODE, NLB, TILE, NL>(a, bi, bj, z, tile, S, smem);
 }
 
 template <int DT, int S, int MODE, int NLB, int TILE>
@@ -941,6 +1078,9 @@ bool decode_prec(int prec, int &dt, int &S) {
   else if (base == OA_PREC_BF16) dt = DT_BF16;
   else if (base == OA_PREC_FP32) dt = DT_F32;
   else return false;
+  // adaptive slice counts: int8 digits, 4..6 slices (the kernels carry the
+  // per chunk level switch only there)
+  if ((prec & OA_PREC_ADAPTIVE) && (dt != DT_I8 || S < ADA_SMIN_MAX)) return false;
   return true;
 }
 
@@ -975,7 +1115,22 @@ int chunk_len(int dt, int S, int n, int K) {
   // enough workgroups for one instance alone: at least ~256 of them
   while (target > 512 && t128 * cdv(K128, (int)target) < 256) target /= 2;
   int kchunk = (int)(target < cap ? target : cap);
-  return max(DBK, kchunk / DBK * DBK);
+  // a multiple of 128, so back-multiply output tiles never straddle chunks
+  return max(128, kchunk / 128 * 128);
+}
+
+// Levels the back-multiply's X psi products keep (MODE 1).
+int backmul_levels(int S, bool lean) {
+  const int nl = lean ? S - 2 : S - 1;
+  return nl < 1 ? 1 : nl;
+}
+
+// The int8 back-multiply contracts over all np padded states in one int32
+// sum per level: level l holds l + 1 products of |digit| <= 127, so the sum
+// stays exact while np * NLB * 127^2 < 2^31 (np < 33,288 at NLB = 4).
+bool backmul_fits(int dt, int S, bool lean, int np) {
+  if (dt != DT_I8) return true;  // 16-bit and fp32 digits promote to FP64 every 128 states
+  return (long)np * backmul_levels(S, lean) * 127L * 127L < 2147483648L;
 }
 
 oa_split_plan make_plan(int prec, int n, int K, int tile) {
@@ -990,6 +1145,7 @@ oa_split_plan make_plan(int prec, int n, int K, int tile) {
   pl.ntiles = T * (T + 1) / 2;
   pl.kchunk = chunk_len(pl.dt, pl.S, n, K);
   pl.nz = cdv(pl.Kp, pl.kchunk);
+  pl.adaptive = (prec & OA_PREC_ADAPTIVE) ? 1 : 0;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t o = off;
@@ -1005,6 +1161,7 @@ oa_split_plan make_plan(int prec, int n, int K, int tile) {
   pl.off_diag = take((size_t)pl.np * 8);
   pl.off_md = take(8);
   pl.off_part = take((size_t)pl.nz * pl.ntiles * tile * tile * 8);
+  pl.off_sl = take((size_t)pl.nz * 4);
   pl.total = off;
   return pl;
 }
@@ -1067,8 +1224,10 @@ void fill_inst(const oa_split_plan &pl, int n, int K, const double *psi, long ld
   const int vec = (((uintptr_t)psi & 15) == 0 && (ld % 2) == 0) ? 1 : 0;
   r->cm = oa_chunkmax_args{psi, ld, n, K, kx0_end, pl.kchunk, pl.nz, 0, maxS, maxB};
   r->items_cm = pl.nz * cdv(n, 8);
+  int *slices = pl.adaptive ? reinterpret_cast<int *>(ws + pl.off_sl) : nullptr;
+  r->cnt = oa_count_args{maxB, slices, pl.nz, K, pl.kchunk, pl.S};
   r->sl = oa_slice_args{psi, ld, n, K, kx0_end, pl.kchunk, maxS, maxB, ws + pl.off_ds,
-                        ws + pl.off_db, pl.np, pl.Kp, 1, vec, 1, 0};
+                        ws + pl.off_db, pl.np, pl.Kp, 1, vec, 1, 0, slices};
   r->items_sl = (pl.Kp / SLT) * (pl.np / SLT);
   oa_digit_args &a = r->d0;
   a = oa_digit_args{};
@@ -1086,6 +1245,7 @@ void fill_inst(const oa_split_plan &pl, int n, int K, const double *psi, long ld
   a.nz = pl.nz;
   a.part = reinterpret_cast<double *>(ws + pl.off_part);
   a.chunk_max = maxS;
+  a.slices = slices;
   r->items_d0 = pl.ntiles * pl.nz;
   r->rd = oa_sreduce_args{a.part, n, pl.nz, alpha, psi, ld, kx0_end, 0, Sout, 0.0, nullptr};
   r->items_rd = cdv(pl.ntiles * tile * tile, 256);
@@ -1117,6 +1277,8 @@ void fill_inst(const oa_split_plan &pl, int n, int K, const double *psi, long ld
   b.Kp = pl.Kp;
   b.colmax = maxB;
   b.kchunk = pl.kchunk;
+  b.nz = pl.nz;
+  b.slices = slices;
   r->items_d1 = cdv(n, tile) * cdv(K, tile);
 }
 
@@ -1126,6 +1288,7 @@ void launch_slice_psi(hipStream_t s, const oa_split_plan &pl, const oa_split_ins
   hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, r.cm.maxS, (long)pl.nz);
   hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, r.cm.maxB, (long)pl.nz);
   OA_LAUNCH_ITEMS(chunk_max_kernel, 256, r.items_cm, s, r.cm);
+  if (pl.adaptive) hipLaunchKernelGGL(slice_count_kernel, dim3(1), dim3(256), 0, s, r.cnt);
   oa_slice_args sl = r.sl;
   sl.want_ds = want_ds ? 1 : 0;
   OA_LAUNCH_ITEMS((slice_psi_kernel<DT, S>), 256, r.items_sl, s, sl);
@@ -1344,6 +1507,9 @@ int grouped_phase(hipStream_t s, int phase, const oa_group &g) {
     case OA_SP_CHUNKMAX:
       OA_LAUNCH_GROUPED(chunk_max_grouped, 256, s, g, oa_chunkmax_args);
       break;
+    case OA_SP_COUNT:
+      OA_LAUNCH_GROUPED(slice_count_grouped, 256, s, g, oa_count_args);
+      break;
     case OA_SP_SLICE:
       OA_LAUNCH_GROUPED((slice_psi_grouped<DT, S>), 256, s, g, oa_slice_args);
       break;
@@ -1394,6 +1560,14 @@ oa_split_plan oa_split_plan_for(int prec, int n, int K, int tile) {
   return make_plan(prec & 0xffff, n, K, tile);
 }
 
+int oa_split_check(int prec, int n, int tile) {
+  int dt, S;
+  if (!decode_prec(prec & 0xffff, dt, S)) return OA_ERR_UNSUPPORTED;
+  const int np = cdv(n, tile) * tile;
+  return backmul_fits(dt, S, (prec & OA_PREC_LEAN_BACKMUL) != 0, np) ? OA_SUCCESS
+                                                                     : OA_ERR_CAPACITY;
+}
+
 void oa_split_inst_fill(const oa_split_plan &pl, int prec, int n, int K, const double *psi,
                         long ld, int kx0_end, double alpha, double *S, const double *T,
                         int transT, double beta, double *out, long ldout, char *ws,
@@ -1413,6 +1587,28 @@ int oa_split_grouped(hipStream_t s, int prec, int tile, int phase, const oa_grou
   if (dt == DT_F16) return grouped_dt<DT_F16>(s, S, lean, tile, phase, g);
   if (dt == DT_F32) return grouped_dt<DT_F32>(s, S, lean, tile, phase, g);
   return grouped_dt<DT_BF16>(s, S, lean, tile, phase, g);
+}
+
+int oa_launch_slice_counts(hipStream_t s, int prec, int n, int K, const double *psi, long ld,
+                           int kx0_end, void *ws, size_t ws_bytes, const int **counts,
+                           int *nz, int *kchunk) {
+  int dt, Sl;
+  if (!decode_prec(prec & 0xffff, dt, Sl) || !(prec & OA_PREC_ADAPTIVE))
+    return OA_ERR_UNSUPPORTED;
+  const oa_split_plan pl = make_plan(prec & 0xffff, n, K, 128);
+  if (pl.total > ws_bytes) return OA_ERR_CAPACITY;
+  oa_split_inst_args r;
+  char *w = (char *)ws;
+  fill_inst(pl, n, K, psi, ld, kx0_end, 1.0, nullptr, nullptr, 0, 0.0, nullptr, 0, w, nullptr,
+            nullptr, &r);
+  hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, r.cm.maxS, (long)pl.nz);
+  hipLaunchKernelGGL(fill_u64, dim3(cdv(pl.nz, 256)), dim3(256), 0, s, r.cm.maxB, (long)pl.nz);
+  OA_LAUNCH_ITEMS(chunk_max_kernel, 256, r.items_cm, s, r.cm);
+  hipLaunchKernelGGL(slice_count_kernel, dim3(1), dim3(256), 0, s, r.cnt);
+  *counts = r.cnt.slices;
+  *nz = pl.nz;
+  *kchunk = pl.kchunk;
+  return hipPeekAtLastError() == hipSuccess ? OA_SUCCESS : OA_ERR_HIP;
 }
 
 int oa_fill_u64(hipStream_t s, unsigned long long *p, long n) {
@@ -1453,6 +1649,7 @@ int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *
                            size_t ws_bytes) {
   int dt, Sl;
   if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   const LambdaPlan lp = make_lambda_plan(prec & 0xffff, n, K);
   if (lp.total > ws_bytes) return OA_ERR_CAPACITY;
   char *w = (char *)ws;
@@ -1465,6 +1662,7 @@ int oa_launch_lambda_split(hipStream_t s, int prec, int n, int K, const double *
 int oa_launch_jswap_db(hipStream_t s, int prec, int n, int K, void *ws, size_t ws_bytes) {
   int dt, Sl;
   if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   const oa_split_plan pl = make_plan(prec & 0xffff, n, K, 128);
   if (pl.total > ws_bytes) return OA_ERR_CAPACITY;
   // DB: S slices of Kp x np digits; Kp and np are multiples of 128, so whole
@@ -1558,6 +1756,7 @@ int oa_launch_zimag_split(hipStream_t s, int prec, int n, int K, double alpha, d
                           void *ws, size_t ws_bytes) {
   int dt, Sl;
   if (!decode_prec(prec & 0xffff, dt, Sl)) return OA_ERR_UNSUPPORTED;
+  if (prec & OA_PREC_ADAPTIVE) return OA_ERR_UNSUPPORTED;  // uniform slices only
   const ZimagPlan zp = make_zimag_plan(prec & 0xffff, n, K);
   if (zp.total > ws_bytes) return OA_ERR_CAPACITY;
   char *w = (char *)ws;
@@ -1577,6 +1776,7 @@ int oa_launch_backmul_split(hipStream_t s, int prec, int n, int K, const double 
   if (pl.total > ws_bytes) return OA_ERR_CAPACITY;
   const bool reuse = (prec & OA_PREC_REUSE_DIGITS) != 0;
   const bool lean = (prec & OA_PREC_LEAN_BACKMUL) != 0;
+  if (!backmul_fits(pl.dt, pl.S, lean, pl.np)) return OA_ERR_CAPACITY;
   char *w = (char *)ws;
   if (dt == DT_I8)
     return dispatch_backmul<DT_I8>(s, pl, lean, n, K, M, transM, psi, ldpsi, kx0_end, reuse,

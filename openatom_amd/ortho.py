@@ -112,6 +112,24 @@ class Handle:
               "oa_overlap_sym")
         return S
 
+    def slice_counts(self, psi: torch.Tensor, nkx0: int = 1, prec="int8x5a"):
+        """Per chunk digit counts an adaptive split precision assigns to psi.
+        Returns (counts as a list, columns per chunk in doubles)."""
+        n = psi.shape[0]
+        _need(psi, "psi")
+        npw = psi.shape[1] // 2
+        nz, kc = ctypes.c_int(0), ctypes.c_int(0)
+        L = _lib.lib()
+        st = L.oa_slice_counts(self.h, n, npw, _ptr(psi), _ld(psi), nkx0, prec_code(prec),
+                               None, 0, ctypes.byref(nz), ctypes.byref(kc))
+        if st != _lib.OA_ERR_CAPACITY:
+            check(st, "oa_slice_counts")
+        buf = (ctypes.c_int * max(nz.value, 1))()
+        check(L.oa_slice_counts(self.h, n, npw, _ptr(psi), _ld(psi), nkx0, prec_code(prec),
+                                buf, nz.value, ctypes.byref(nz), ctypes.byref(kc)),
+              "oa_slice_counts")
+        return list(buf[:nz.value]), kc.value
+
     def lambda_fwd(self, psi: torch.Tensor, F: torch.Tensor, alpha: float = 1.0,
                    prec="fp64", L: torch.Tensor | None = None) -> torch.Tensor:
         n = psi.shape[0]
@@ -151,8 +169,9 @@ class Handle:
     def backmul(self, M: torch.Tensor, psi: torch.Tensor, transM: bool = False,
                 alpha: float = 1.0, beta: float = 0.0, out: torch.Tensor | None = None,
                 prec="fp64", reuse_digits: bool = False) -> torch.Tensor:
-        """reuse_digits: take psi's digit slices from the preceding overlap_sym
-        of the same psi (split precisions only)."""
+        """reuse_digits: take psi's digit slices from the preceding split
+        slicing of the same psi on this handle (split precisions only); psi is
+        sliced again when the workspace no longer holds them."""
         n = psi.shape[0]
         _need(psi, "psi")
         _need(M, "M", n)
@@ -195,14 +214,17 @@ class Handle:
         return M
 
     def zoverlap(self, psi: torch.Tensor, alpha: float = 1.0, prec="fp64",
-                 S: torch.Tensor | None = None) -> torch.Tensor:
-        """S = alpha psi psi^H as an n x n complex128 tensor."""
+                 S: torch.Tensor | None = None, diag: float = 0.0,
+                 maxdev: torch.Tensor | None = None) -> torch.Tensor:
+        """S = alpha psi psi^H as an n x n complex128 tensor; `maxdev` (a
+        1-element float64 device tensor) receives array_diag_max of S."""
         _need(psi, "psi")
         n = psi.shape[0]
         if S is None:
             S = torch.empty((n, n), dtype=torch.complex128, device=psi.device)
         check(_lib.lib().oa_zoverlap(self.h, n, psi.shape[1] // 2, _ptr(psi), _ld(psi), alpha,
-                                     S.data_ptr(), prec_code(prec)), "oa_zoverlap")
+                                     S.data_ptr(), prec_code(prec), diag, _ptr(maxdev)),
+              "oa_zoverlap")
         return S
 
     def zlambda(self, psi: torch.Tensor, F: torch.Tensor, alpha: float = 1.0,
@@ -294,6 +316,60 @@ class Handle:
         if sync:
             return S, T, out, it.value, err.value
         return S, T, out, None, None
+
+    # ------------------------------------------------------------ transport
+    @staticmethod
+    def _msg_table(msgs, block=None):
+        for m in msgs:
+            r0, nr, c0, nc, buf = m[:5]
+            if buf.numel() != nr * nc:
+                raise ValueError(f"message buffer holds {buf.numel()} doubles, not {nr} x {nc}")
+            if buf.dtype != torch.float64:
+                raise TypeError("message buffers must be float64")
+            if block is not None and (c0 < 0 or c0 + nc > block.shape[1]):
+                raise ValueError("invalid argument #5: message columns outside the block")
+        count = len(msgs)
+        ci, cl = ctypes.c_int * max(count, 1), ctypes.c_long * max(count, 1)
+        return (count, ci(*[m[0] for m in msgs]), ci(*[m[1] for m in msgs]),
+                cl(*[m[2] for m in msgs]), ci(*[m[3] for m in msgs]))
+
+    def collate_rows(self, block: torch.Tensor, msgs) -> torch.Tensor:
+        """MessageDataCollator::operator() (MessageDataCollator.h:163-247) for a
+        list of messages (row0, nrows, col0, ncols, data): each data tensor
+        (nrows x ncols float64, host or device, all on the same side) is
+        copied into block[row0:row0+nrows, col0:col0+ncols]."""
+        _need(block, "block")
+        count, r0, nr, c0, nc = self._msg_table(msgs, block)
+        datas = [m[4].contiguous() for m in msgs]
+        on_host = bool(datas) and not datas[0].is_cuda
+        if any(d.is_cuda == on_host for d in datas):
+            raise ValueError("message buffers must all be host or all be device tensors")
+        ptrs = (ctypes.c_void_p * max(count, 1))(*[d.data_ptr() for d in datas])
+        check(_lib.lib().oa_collate_rows(self.h, count, r0, nr, c0, nc, ptrs, int(on_host),
+                                         _ptr(block), _ld(block), block.shape[0]),
+              "oa_collate_rows")
+        return block
+
+    def scatter_rows(self, block: torch.Tensor, msgs) -> None:
+        """The return leg of the backward path (sendBWResultDirect,
+        ckPairCalculator.C:2365-2452, unpacked by CP_State_GSpacePlane.C:
+        3120-3126 / 3197-3203): msgs are (row0, nrows, col0, ncols, dst,
+        accumulate); dst (nrows x ncols float64, host or device) receives
+        block[row0:row0+nrows, col0:col0+ncols], copied or added (fastAdd)."""
+        _need(block, "block")
+        count, r0, nr, c0, nc = self._msg_table(msgs, block)
+        dsts = [m[4] for m in msgs]
+        for d in dsts:
+            if not d.is_contiguous():
+                raise ValueError("destination buffers must be contiguous")
+        on_host = bool(dsts) and not dsts[0].is_cuda
+        if any(d.is_cuda == on_host for d in dsts):
+            raise ValueError("destinations must all be host or all be device tensors")
+        ptrs = (ctypes.c_void_p * max(count, 1))(*[d.data_ptr() for d in dsts])
+        acc = (ctypes.c_int * max(count, 1))(*[1 if m[5] else 0 for m in msgs])
+        check(_lib.lib().oa_scatter_rows(self.h, count, r0, nr, c0, nc, ptrs, int(on_host), acc,
+                                         _ptr(block), _ld(block), block.shape[0]),
+              "oa_scatter_rows")
 
 
 class Batch:
@@ -441,12 +517,12 @@ class PairCalculator:
                 if right is None:
                     raise ValueError("asymmetric instance needs the right (force) data")
                 return h.zlambda(left, right, prec=self.prec)
-            S = h.zoverlap(left, prec=self.prec)
-            if diag is None:
-                return S
+            md = None
+            if diag is not None:
+                md = torch.zeros(1, dtype=torch.float64, device=left.device)
             # Ortho::array_diag_max on the complex S (ortho.h:282-313, myabs = |z|)
-            md = (S - diag * torch.eye(S.shape[0], dtype=S.dtype, device=S.device)).abs().max()
-            return S, float(md.item())
+            S = h.zoverlap(left, prec=self.prec, diag=diag or 0.0, maxdev=md)
+            return S if md is None else (S, float(md.item()))
         if self.symmetric:
             alpha = 2.0 if self.double_pack else 1.0
             md = None

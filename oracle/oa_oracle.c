@@ -54,6 +54,9 @@
  *                        squares), ZGEMM('N','N') / ('N','C') backward
  *                        (:1792-1842).  Every GEMM goes through a zgemm
  *                        pointer (oao_zgemm or the compiled zgemm_).
+ *   oao_array_diag_max   Ortho::array_diag_max (ortho.h:282-313) per Ortho
+ *                        tile and the max_double reduction into maxCheck
+ *                        (ortho.C:255-262, 455-475), real or complex S.
  *   oao_gsp_unpack       the GSpace accumulation of the backward result into
  *                        the force: unpackLambda(partialResultMsg)
  *                        CP_State_GSpacePlane.C:2168-2195, unpackLambda(
@@ -704,6 +707,49 @@ int oao_zortho_step(int n, int npw, const double *psi, int nspin, double tol, in
   memset(psi_out, 0, (size_t)2 * n * npw * sizeof(double));
   oao_zbackmul(n, npw, T, 0, psi, 1.0, 0.0, psi_out, zg);
   return st;
+}
+
+/* Ortho::array_diag_max (orthog_ctrl/ortho.h:282-313) over the og x og
+ * Ortho tiles of a row-major n x n S, combined by the max_double reduction
+ * that feeds Ortho::maxCheck (ortho.C:255-262, 455-475).  Off-diagonal tiles:
+ * max |S_ij| from the tile's first element on.  Diagonal tiles: the first
+ * value is |S_00 - d| and every later one is |S_ij|, with ||S_ii| - d| on the
+ * diagonal; d = 1 under lsda, else 2.  complex != 0 reads S as row-major
+ * complex (re, im) pairs and takes myabs = std::abs, the modulus. */
+static double dmodulus(const double *S, size_t idx, int cplx) {
+  return cplx ? hypot(S[2 * idx], S[2 * idx + 1]) : fabs(S[idx]);
+}
+
+double oao_array_diag_max(int n, const double *S, int og, int lsda, int cplx) {
+  const double diag = lsda ? 1.0 : 2.0;
+  double result = 0.0;
+  int first = 1;
+  if (og < 1) og = n;
+  for (int x0 = 0; x0 < n; x0 += og)
+    for (int y0 = 0; y0 < n; y0 += og) {
+      const int m = (x0 + og <= n) ? og : n - x0, w = (y0 + og <= n) ? og : n - y0;
+      const size_t i00 = (size_t)x0 * n + y0;
+      double mx;
+      if (x0 != y0) {
+        mx = dmodulus(S, i00, cplx);
+        for (int i = 0; i < m; i++)
+          for (int j = 0; j < w; j++) {
+            const double a = dmodulus(S, (size_t)(x0 + i) * n + y0 + j, cplx);
+            mx = (mx > a) ? mx : a;
+          }
+      } else {
+        mx = cplx ? hypot(S[2 * i00] - diag, S[2 * i00 + 1]) : fabs(S[i00] - diag);
+        for (int i = 0; i < m; i++)
+          for (int j = 0; j < w; j++) {
+            double a = dmodulus(S, (size_t)(x0 + i) * n + y0 + j, cplx);
+            if (i == j) a = fabs(a - diag);
+            mx = (mx > a) ? mx : a;
+          }
+      }
+      if (first || mx > result) result = mx;
+      first = 0;
+    }
+  return result;
 }
 
 /* max |S(psi) - occ*I| with S in the 2Re - G0 metric of oao_overlap. */

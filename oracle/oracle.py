@@ -96,6 +96,8 @@ def lib() -> ctypes.CDLL:
                                       _vp]
         L.oao_gsp_unpack.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.oao_array_diag_max.restype = _d
+        L.oao_array_diag_max.argtypes = [_i, _dp, _i, _i, _i]
         _lib = L
     return _lib
 
@@ -322,6 +324,16 @@ def gsp_unpack(data: np.ndarray, force: np.ndarray, nkx0: int = 1, double_pack: 
     lib().oao_gsp_unpack(n, K // 2, _p(data), nkx0, 1 if double_pack else 0, mode,
                          1 if lsda else 0, _p(force))
     return force
+
+
+def array_diag_max(S: np.ndarray, lsda: bool = False, og: int = 0) -> float:
+    """Ortho::array_diag_max over og x og Ortho tiles (og = 0: one tile),
+    reduced with max_double as maxCheck receives it; complex S takes the
+    modulus (myabs = std::abs)."""
+    n = S.shape[0]
+    cplx = np.iscomplexobj(S)
+    A = np.ascontiguousarray(S.view(np.float64) if cplx else S, dtype=np.float64)
+    return float(lib().oao_array_diag_max(n, _p(A), og, 1 if lsda else 0, 1 if cplx else 0))
 
 
 def orthonormality_residual(psi: np.ndarray, nkx0: int = 1, occ: float = 1.0,

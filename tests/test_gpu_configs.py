@@ -1,0 +1,180 @@
+"""The benchmark configurations at their stated sizes.
+
+C  large single instance: n=1024, npw=262,144 (psi 4.3 GB), eps=1e-3, seed 0,
+   plus the Lambda product against a force block F of the same shape.
+D  64 independent instances, n in {64, 96, ..., 512}, npw in [8192, 65536]
+   (multiples of 256), eps log-uniform in [1e-4, 1e-2] (the benchmark's own
+   generator), one grouped step.
+E  a Poisson stream of serving sessions with an n=1024 session among them.
+At these sizes the oracle is too slow, so the checks are size-independent:
+exact symmetry of S, S against torch's own FP64 product, T S T = I,
+orthonormality of T psi, and the equality of grouped and single results.
+The small-size checks against the oracle are in test_gpu_f64.py,
+test_gpu_adaptive.py and at the end of this file.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from openatom_amd.workload import make_force, make_states, metric_weights  # noqa: E402
+
+DEV = "cuda"
+
+
+def rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+def resid(handle, out):
+    n = out.shape[0]
+    S = handle.overlap_sym(out, nkx0=1, prec="fp64")
+    return (S - torch.eye(n, dtype=torch.float64, device=DEV)).abs().max().item()
+
+
+@pytest.fixture(scope="module")
+def large():
+    psi = make_states(1024, 262144, eps=1e-3, seed=0, device=DEV, rng="torch")
+    torch.cuda.synchronize()
+    yield psi
+    del psi
+    torch.cuda.empty_cache()
+
+
+@pytest.fixture(scope="module")
+def large_fp64(handle, large):
+    S, T, out, it, err = handle.ortho_step(large, nkx0=1, nspin=2, tol=1e-12, maxit=50)
+    torch.cuda.synchronize()
+    return S, T, out, it
+
+
+def test_config_c_fp64_step(handle, large, large_fp64):
+    S, T, out, it = large_fp64
+    n = 1024
+    eye = torch.eye(n, dtype=torch.float64, device=DEV)
+    assert torch.equal(S, S.T)
+    w = torch.from_numpy(metric_weights(262144, 1)).to(DEV)
+    S_ref = (large * w) @ large.T
+    e_s = rel(S, S_ref)
+    tst = (T @ S @ T - eye).abs().max().item()  # nspin 2: B = S, T = S^(-1/2)
+    r = resid(handle, out)
+    print(f"config C fp64: {it} iterations, |S - torch| {e_s:.2e}, |TST - I| {tst:.2e}, "
+          f"residual {r:.2e}")
+    assert e_s < 1e-12 and tst < 1e-12 and r <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["int8x5", "int8x5a"])
+def test_config_c_split_step(handle, large, large_fp64, prec):
+    S64, T64, _, _ = large_fp64
+    S, T, out, it, err = handle.ortho_step(large, nkx0=1, nspin=2, tol=1e-12, maxit=50,
+                                           prec=prec)
+    torch.cuda.synchronize()
+    assert torch.equal(S, S.T)
+    e_s, dT, r = rel(S, S64), (T - T64).abs().max().item(), resid(handle, out)
+    print(f"config C {prec}: |S - S64| {e_s:.2e}, |T - T64| {dT:.2e}, residual {r:.2e}")
+    assert e_s < 6e-10 and dT <= 1e-10 and r <= 1e-10
+    del S, T, out
+
+
+def test_config_c_lambda(handle, large):
+    """Lambda = psi F^T against F of the same shape (i.i.d. complex normal,
+    std 0.1): FP64 against torch's product, int8x5 against FP64 within ten
+    times the uniform int8x5 overlap tolerance (F carries no envelope)."""
+    F = make_force(1024, 262144, seed=1, device=DEV, rng="torch")
+    L64 = handle.lambda_fwd(large, F, alpha=1.0, prec="fp64")
+    L_ref = large @ F.T
+    Ls = handle.lambda_fwd(large, F, alpha=1.0, prec="int8x5")
+    torch.cuda.synchronize()
+    e64, es = rel(L64, L_ref), rel(Ls, L64)
+    print(f"config C Lambda: fp64 vs torch {e64:.2e}, int8x5 vs fp64 {es:.2e}")
+    assert e64 < 1e-12 and es < 3e-9
+    del F, L_ref
+
+
+@pytest.mark.parametrize("prec", ["int8x5", "int8x5a"])
+def test_config_d_batched64(handle, prec):
+    """The benchmark's 64 uneven instances in one grouped step (graph replay):
+    every instance orthonormal to 1e-10, a sample bit-identical to the
+    single-instance step."""
+    import bench
+    from openatom_amd.ortho import Batch
+    psis = bench.batched_instances()
+    b = Batch(handle)
+    b.plan(psis, nkx0=1, nspin=2, tol=1e-12, maxit=12, prec=prec)
+    b.launch()
+    b.launch()
+    torch.cuda.synchronize()
+    reps = b.reports()
+    worst = max(resid(handle, o) for o in b.outs)
+    print(f"config D {prec}: max residual {worst:.2e}, max iterations "
+          f"{max(r[0] for r in reps)}")
+    assert all(r[2] == 0 for r in reps)
+    assert worst <= 1e-10
+    for i in (0, 17, 40, 63):
+        _, _, out, _, _ = handle.ortho_step(psis[i], nkx0=1, nspin=2, tol=1e-12, maxit=12,
+                                            prec=prec)
+        torch.cuda.synchronize()
+        assert torch.equal(out, b.outs[i]), i
+    b.close()
+    del psis
+    torch.cuda.empty_cache()
+
+
+def test_config_e_poisson_stream_with_n1024(handle):
+    """A Poisson stream (offered 400 req/s, 4 MD steps per session) holding an
+    n=1024 session: results orthonormal to 1e-10 and, for a sample including
+    the n=1024 session, the same alone as in the mix, bit for bit."""
+    from openatom_amd.serve import GpuExecutor, poisson_sessions, serve
+    from openatom_amd.workload import md_perturbation, md_step_
+    steps = 4
+    seed = next(s for s in range(1000)
+                if any(sp.n == 1024 for sp in poisson_sessions(400.0, 0.25, steps=steps,
+                                                                seed=s)))
+    specs = poisson_sessions(400.0, 0.25, steps=steps, seed=seed)
+    ex = GpuExecutor(handle, prec="int8x5", tol=1e-12, maxit=12, record=True)
+    r = serve(specs, ex, budget_bytes=120 * 2**30)
+    ex.release()
+    print(f"config E: {len(specs)} sessions (seed {seed}), mean batch {r['mean_batch']:.1f}, "
+          f"p50 {r['latency_p50_ms']} ms, p99 {r['latency_p99_ms']} ms")
+    assert r["sessions_finished"] == len(specs)
+    assert r["requests_completed"] == steps * len(specs)
+    big = [sp for sp in specs if sp.n == 1024]
+    sample = big[:1] + [sp for sp in specs if sp.n != 1024][:2]
+    for sp in specs:
+        assert resid(handle, ex.records[sp.sid][-1]) <= 1e-10, sp.sid
+    for sp in sample:
+        psi = make_states(sp.n, sp.npw, eps=1e-3, seed=sp.seed, device=DEV, rng="torch")
+        P = md_perturbation(sp.n, sp.npw, seed=sp.seed * 1000 + 1)
+        for k in range(steps):
+            _, _, out, _, _ = handle.ortho_step(psi, nkx0=1, nspin=2, tol=1e-12, maxit=12,
+                                                prec="int8x5")
+            torch.cuda.synchronize()
+            assert torch.equal(out, ex.records[sp.sid][k]), (sp.sid, k)
+            psi = torch.empty_like(out)
+            md_step_(psi, out, P, 1e-3, k + 1)
+    ex.records.clear()
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("int8x5", 3e-10), ("int8x5a", 6e-10)])
+def test_small_batch_against_the_oracle(handle, oracle, prec, tol):
+    """A grouped batch of three small uneven instances against the oracle's
+    step (netlib DGEMM order, the Ortho loop): S, T and T psi, FP64 within
+    1e-12 relative, the splits within their overlap tolerance."""
+    from openatom_amd.ortho import Batch
+    shapes = [(40, 1200, 1e-3), (64, 2048, 1e-2), (24, 700, 1e-4)]
+    hs = [make_states(n, p, eps=e, seed=50 + i) for i, (n, p, e) in enumerate(shapes)]
+    b = Batch(handle)
+    b.plan([h.to(DEV) for h in hs], nkx0=1, nspin=2, tol=1e-12, maxit=30, prec=prec)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, h in enumerate(hs):
+        ref = oracle.ortho_step(h.numpy(), nkx0=1, nspin=2, tol=1e-12, maxit=30)
+        assert ref["status"] == 0
+        for name, got, want in (("S", b.S[i], ref["S"]), ("T", b.T[i], ref["T"]),
+                                ("T psi", b.outs[i], ref["psi_out"])):
+            g = got.cpu().numpy()
+            e = float(np.abs(g - want).max() / np.abs(want).max())
+            assert e < tol, (i, name, e)
+    b.close()

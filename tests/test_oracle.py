@@ -513,3 +513,38 @@ def test_zortho_step_reference_rule_under_lsda(oracle):
     psi = make_kpoint_states(30, 400, eps=1e-2, seed=8).numpy()
     res = oracle.zortho_step(psi, nspin=2, tol=1e-15, maxit=50)
     assert res["status"] == 0 and res["iters"] == 1 and np.isnan(res["err"])
+
+
+# ------------------------------------------------------------ array_diag_max
+def test_array_diag_max_pins(oracle):
+    """Ortho::array_diag_max (ortho.h:282-313) reduced over the Ortho tiles:
+    - for S with a non-negative diagonal it is max|S - d I| (numpy), for every
+      tile size og (the max reduction makes the tiling invisible);
+    - d = 1 under lsda, 2 otherwise;
+    - on a diagonal tile the diagonal term is ||S_ii| - d|, so a negative
+      diagonal entry -x counts as |x - d|, not x + d; the tile's first element
+      also enters as |S_00 - d|, which only matters when S_00 < 0;
+    - complex S takes the modulus (myabs = std::abs)."""
+    rng = np.random.default_rng(3)
+    n = 11
+    X = rng.standard_normal((n, n)) * 1e-3
+    S = 2.0 * np.eye(n) + (X + X.T)
+    want = np.abs(S - 2.0 * np.eye(n)).max()
+    for og in (0, 1, 3, 4, 11):
+        assert oracle.array_diag_max(S, og=og) == want
+    assert oracle.array_diag_max(S, lsda=True) == np.abs(S - np.eye(n)).max()
+    # a negative diagonal entry in a tile that does not start with it
+    S2 = S.copy()
+    S2[5, 5] = -2.5
+    got = oracle.array_diag_max(S2, og=0)
+    assert got == max(np.abs(S2 - 2.0 * np.eye(n))[np.arange(n) != 5].max(),
+                      np.abs(S2[5, np.arange(n) != 5]).max(), abs(2.5 - 2.0))
+    # ... and as the first element of a tile: |S_00 - d| = 4.5 enters too
+    S3 = S.copy()
+    S3[0, 0] = -2.5
+    assert oracle.array_diag_max(S3, og=0) == 4.5
+    # complex: modulus
+    Y = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 1e-3
+    Z = np.eye(n) + (Y + Y.conj().T)
+    zwant = np.abs(Z - np.eye(n)).max()
+    assert abs(oracle.array_diag_max(Z, lsda=True) - zwant) <= 1e-18
