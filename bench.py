@@ -55,8 +55,8 @@ def parse():
                     help="serving: seconds of session arrivals per rate")
     ap.add_argument("--serve-steps", type=int, default=100,
                     help="serving: MD steps (requests) per session")
-    ap.add_argument("--modes", default="fp64,int8x5,int8x5a",
-                    help="comma list of precisions to run (fp64, int8x5, int8x5a, fp16x3, ...)")
+    ap.add_argument("--modes", default="fp64,int8x5,int8x6a",
+                    help="comma list of precisions to run (fp64, int8x5, int8x6a, fp16x3, ...)")
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--maxit", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -65,10 +65,13 @@ extern "C" {
 /* Adaptive slice counts (OA_PREC_INT8 with 4 <= s <= 6): every K chunk z
  * gets its own count s_z in [2, s], chosen on the device from the chunk
  * maxima alone.  With W_z = K_z 4^(e_z) (K_z columns, chunk exponent e_z)
- * and W = sum_z W_z, chunk z drops d slices when W_z 254^d <= W / nz.  The
- * element-wise truncation bound of S (and of the residual after T psi) is
- * at most twice the bound of s uniform slices; chunks whose coefficients are
- * small next to the largest chunk run 2-4 slices instead of s.  Applies to
+ * and W = sum_z W_z, chunk z drops d slices while
+ * W_z 254^d <= (254 / 4) W / nz.  The element-wise truncation bound of S
+ * (and of the residual after T psi) is then at most 64.5 times the bound of
+ * s uniform slices, i.e. about a quarter of the bound of s - 1 uniform
+ * slices: OA_PREC_SLICES(OA_PREC_INT8, 6) | OA_PREC_ADAPTIVE is more
+ * accurate than 5 uniform slices, and chunks whose coefficients are small
+ * next to the largest chunk run 2-5 slices instead of 6.  Applies to
  * oa_overlap_sym, oa_backmul, oa_lambda_force, oa_ortho_step and batches;
  * the Lambda product and the complex calls return OA_ERR_UNSUPPORTED. */
 #define OA_PREC_ADAPTIVE (1 << 13)

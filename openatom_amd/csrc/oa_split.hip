@@ -66,6 +66,7 @@ constexpr int DBK = 32;   // chunk granularity (elements); a stage holds 32 byte
 constexpr int SMAX = 6;
 constexpr int SLT = 64;   // slicing tile (rows x columns)
 constexpr int ADA_SMIN = 2;      // fewest slices an adaptive chunk keeps
+constexpr double ADA_THETA = 254.0 / 4.0;  // budget of the dropped chunks, in units of B_S
 constexpr int ADA_SMIN_MAX = 4;  // adaptive mode needs at least this many slices at most
 constexpr long KCHUNK_TARGET = 8192;  // columns per scale block / split-K chunk
 
@@ -319,11 +320,13 @@ __global__ __launch_bounds__(256) void chunk_max_grouped(const oa_chunkmax_args 
 // (kept levels p + q < s_z plus the remainders of s_z digits; c_s grows
 // slowly with s), and the same weight W_z = K_z 4^(e_z) prices the error the
 // chunk's columns of T psi add to the new states' overlap.  With
-// W = sum_z W_z, chunk z drops d slices when W_z 254^d <= W / nz, so the
-// dropped chunks add at most W 254^-(S-1) c_S: the bound stays within twice
-// that of S uniform slices.  Weights are taken relative to the largest
-// exponent, and both sums run in a fixed order, so every launch form gives
-// the same counts.
+// W = sum_z W_z, chunk z drops d slices while W_z 254^d <= ADA_THETA W / nz.
+// The kept chunks add at most the bound B_S of S uniform slices, the dropped
+// ones at most ADA_THETA B_S, so the total stays within (1 + ADA_THETA) B_S;
+// with ADA_THETA = 254 / 4 that is a quarter of the bound of S - 1 uniform
+// slices (plus B_S).  Weights are taken relative to the largest exponent,
+// and both sums run in a fixed order, so every launch form gives the same
+// counts.
 __device__ __forceinline__ double chunk_weight(const oa_count_args &a, int z, int eref) {
   const double m = __longlong_as_double((long long)a.maxB[z]);
   if (!(m > 0.0)) return 0.0;
@@ -356,7 +359,7 @@ __device__ __forceinline__ void slice_count_body(const oa_count_args &a) {
     if (t < o) red[t] += red[t + o];
     __syncthreads();
   }
-  const double budget = red[0] / (double)a.nz;
+  const double budget = ADA_THETA * red[0] / (double)a.nz;
   for (int z = t; z < a.nz; z += 256) {
     double wz = chunk_weight(a, z, eref);
     int s = a.smax;
