@@ -185,7 +185,9 @@ int oa_state_read(const char *path, int format, int double_pack, int capacity, d
   int ncoef, nx, ny, nz;
   if (!src.header(ncoef, nx, ny, nz) || ncoef < 0) return OA_ERR_IO;
   std::vector<Rec> recs;
-  recs.reserve((size_t)ncoef);
+  // the header is not trusted for the allocation: a damaged count fails at
+  // the first missing record instead of reserving gigabytes
+  recs.reserve((size_t)std::min(ncoef, 1 << 20));
   for (int i = 0; i < ncoef; i++) {
     Rec r;
     if (!src.record(r)) return OA_ERR_IO;
