@@ -210,7 +210,7 @@ class GpuExecutor:
     """Device side of the loop: resident coefficients per session and one
     grouped Batch over all admitted sessions."""
 
-    def __init__(self, handle=None, prec="int8x5", eps: float = 1e-3, nspin: int = 2,
+    def __init__(self, handle=None, prec="int8x6a", eps: float = 1e-3, nspin: int = 2,
                  tol: float = 1e-12, maxit: int = 12, use_graph: bool = True,
                  record: bool = False, reserve_bytes: int = 0):
         import torch

@@ -121,7 +121,8 @@ def test_config_d_batched64(handle, prec):
     torch.cuda.empty_cache()
 
 
-def test_config_e_poisson_stream_with_n1024(handle):
+@pytest.mark.parametrize("prec", ["int8x6a"])
+def test_config_e_poisson_stream_with_n1024(handle, prec):
     """A Poisson stream (offered 400 req/s, 4 MD steps per session) holding an
     n=1024 session: results orthonormal to 1e-10 and, for a sample including
     the n=1024 session, the same alone as in the mix, bit for bit."""
@@ -132,7 +133,7 @@ def test_config_e_poisson_stream_with_n1024(handle):
                 if any(sp.n == 1024 for sp in poisson_sessions(400.0, 0.25, steps=steps,
                                                                 seed=s)))
     specs = poisson_sessions(400.0, 0.25, steps=steps, seed=seed)
-    ex = GpuExecutor(handle, prec="int8x5", tol=1e-12, maxit=12, record=True)
+    ex = GpuExecutor(handle, prec=prec, tol=1e-12, maxit=12, record=True)
     r = serve(specs, ex, budget_bytes=120 * 2**30)
     ex.release()
     print(f"config E: {len(specs)} sessions (seed {seed}), mean batch {r['mean_batch']:.1f}, "
@@ -141,14 +142,17 @@ def test_config_e_poisson_stream_with_n1024(handle):
     assert r["requests_completed"] == steps * len(specs)
     big = [sp for sp in specs if sp.n == 1024]
     sample = big[:1] + [sp for sp in specs if sp.n != 1024][:2]
-    for sp in specs:
-        assert resid(handle, ex.records[sp.sid][-1]) <= 1e-10, sp.sid
+    res = {sp.sid: max(resid(handle, o) for o in ex.records[sp.sid]) for sp in specs}
+    worst = max(res, key=res.get)
+    print(f"config E {prec}: worst residual {res[worst]:.2e} (session {worst}, "
+          f"n={specs[worst].n})")
+    assert res[worst] <= 1e-10, (worst, res[worst])
     for sp in sample:
         psi = make_states(sp.n, sp.npw, eps=1e-3, seed=sp.seed, device=DEV, rng="torch")
         P = md_perturbation(sp.n, sp.npw, seed=sp.seed * 1000 + 1)
         for k in range(steps):
             _, _, out, _, _ = handle.ortho_step(psi, nkx0=1, nspin=2, tol=1e-12, maxit=12,
-                                                prec="int8x5")
+                                                prec=prec)
             torch.cuda.synchronize()
             assert torch.equal(out, ex.records[sp.sid][k]), (sp.sid, k)
             psi = torch.empty_like(out)
