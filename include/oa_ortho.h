@@ -246,6 +246,24 @@ int oa_ortho_step(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
                   double *S, double *T, double *psi_out, long ldout,
                   int *iters, double *err);
 
+/* oa_ortho_step in FP64 (prec must be OA_PREC_FP64) with no device memory
+ * allocated by the call or by the library on its behalf.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * buffer beyond the arguments lives in one scratch array of
+ * oa_noalloc_scratch_bytes() (64 MiB) that each device holds for the life of
+ * the process: the S->T report and buffers (5 n^2 doubles) and the overlap's
+ * split-K partial tiles, whose depth is capped to what is left (S, T and
+ * psi_out then differ from oa_ortho_step's by k-sum rounding only).  Sizes
+ * whose buffers do not fit (n above about 1300) return OA_ERR_CAPACITY;
+ * npw must be at least 1.  Calls on one device share the array: calls that
+ * overlap in time on one device race. */
+int oa_ortho_step_noalloc(oa_handle_t h, int n, int npw, const double *psi, long ldpsi,
+                          int nkx0, int nspin, double tol, int maxit, int prec,
+                          double *S, double *T, double *psi_out, long ldout,
+                          int *iters, double *err);
+size_t oa_noalloc_scratch_bytes(void);
+
 /* ---- complex-math (k-point) path --------------------------------------
  *
  * The PairCalculator built with CP_PAIRCALC_USES_COMPLEX_MATH

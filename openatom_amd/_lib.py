@@ -130,6 +130,9 @@ def lib() -> ctypes.CDLL:
          ctypes.POINTER(_i), ctypes.POINTER(_d), _i)
     _sig(L.oa_ortho_step, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp, _l,
          ctypes.POINTER(_i), ctypes.POINTER(_d))
+    _sig(L.oa_ortho_step_noalloc, _i, _vp, _i, _i, _vp, _l, _i, _i, _d, _i, _i, _vp, _vp, _vp,
+         _l, ctypes.POINTER(_i), ctypes.POINTER(_d))
+    _sig(L.oa_noalloc_scratch_bytes, ctypes.c_size_t)
     _dp = ctypes.POINTER(ctypes.c_double)
     _sig(L.oa_zgemm, _i, _vp, ctypes.c_char, ctypes.c_char, _i, _i, _i, _dp, _vp, _i, _vp, _i,
          _dp, _vp, _i)

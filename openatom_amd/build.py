@@ -25,7 +25,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-I", os.path.join(HERE, "..", "include")]
 
 SOURCES = ["oa_f64.hip", "oa_split.hip", "oa_batch.hip", "oa_complex.hip", "oa_kpoint.hip",
-           "oa_statefile.hip", "oa_transport.hip", "oa_api.hip"]
+           "oa_statefile.hip", "oa_transport.hip", "oa_noalloc.hip", "oa_api.hip"]
 
 
 def _compile(src: str) -> str:

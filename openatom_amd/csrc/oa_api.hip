@@ -149,6 +149,11 @@ int oa_create(oa_handle_t *out, int device) {
     return OA_ERR_HIP;
   }
   c->stream = c->own_stream;
+  const int st = oa_noalloc_init(device);
+  if (st) {
+    oa_destroy(c);
+    return st;
+  }
   *out = c;
   return OA_SUCCESS;
 }
@@ -439,30 +444,34 @@ int oa_lambda(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, cons
                                c->ws.bytes);
 }
 
-static int s_to_t_sync(oa_context *c, int n, const double *S, int nspin, double tol, int maxit,
-                       double *T, int *iters, double *err, void *ws) {
-  int st = oa_ns_begin(c->stream, n, S, nspin, c->report, ws);
+}  // extern "C"
+
+int oa_s_to_t_sync(oa_context *c, int n, const double *S, int nspin, double tol, int maxit,
+                   double *T, int *iters, double *err, void *ws, oa_ns_report *rep) {
+  int st = oa_ns_begin(c->stream, n, S, nspin, rep, ws);
   if (st) return st;
   // Host loop with early exit: two guarded iterations per report read.
   int it = 0;
   for (;;) {
     const int upto = (it + 2 < maxit) ? it + 2 : maxit;
     for (; it < upto; it++) {
-      st = oa_ns_iterate(c->stream, n, it, tol, maxit, c->report, ws);
+      st = oa_ns_iterate(c->stream, n, it, tol, maxit, rep, ws);
       if (st) return st;
     }
-    OA_HIP_CHECK(hipMemcpyAsync(c->host_report, c->report, sizeof(oa_ns_report),
+    OA_HIP_CHECK(hipMemcpyAsync(c->host_report, rep, sizeof(oa_ns_report),
                                 hipMemcpyDeviceToHost, c->stream));
     OA_HIP_CHECK(hipStreamSynchronize(c->stream));
     if (c->host_report->done || it >= maxit) break;
   }
-  st = oa_ns_finish(c->stream, n, T, c->report, ws);
+  st = oa_ns_finish(c->stream, n, T, rep, ws);
   if (st) return st;
   OA_HIP_CHECK(hipStreamSynchronize(c->stream));
   if (iters) *iters = c->host_report->iterations;
   if (err) *err = c->host_report->error;
   return c->host_report->done ? c->host_report->status : OA_ERR_NOT_CONVERGED;
 }
+
+extern "C" {
 
 int oa_s_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int maxit,
               double *T, int *iters, double *err) {
@@ -476,7 +485,7 @@ int oa_s_to_t(oa_handle_t c, int n, const double *S, int nspin, double tol, int 
   DeviceGuard g(c->device);
   int st = oa_ws_require_into(c, c->ns, oa_ns_ws_bytes(n));
   if (st) return st;
-  return s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ns.ptr);
+  return oa_s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ns.ptr, c->report);
 }
 
 int oa_s_to_t_async(oa_handle_t c, int n, const double *S, int nspin, double tol, int maxit,
@@ -606,7 +615,8 @@ int oa_ortho_step(oa_handle_t c, int n, int npw, const double *psi, long ldpsi, 
   // the split back-multiply reuses the digits the overlap just produced
   const int bprec = ((prec & 0xff) == OA_PREC_FP64) ? prec : (prec | OA_PREC_REUSE_DIGITS);
   if (iters || err) {
-    const int st_ns = s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ns.ptr);
+    const int st_ns = oa_s_to_t_sync(c, n, S, nspin, tol, maxit, T, iters, err, c->ns.ptr,
+                                     c->report);
     if (st_ns != OA_SUCCESS && st_ns != OA_ERR_NOT_CONVERGED) return st_ns;
     st = oa_backmul(c, n, npw, T, 0, psi, ldpsi, 1.0, 0.0, psi_out, ldout, bprec);
     return st ? st : st_ns;

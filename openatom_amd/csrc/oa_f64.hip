@@ -685,7 +685,10 @@ int ovl_depth(int ntiles, int K, int wpc) {
   return best;
 }
 
-OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym) {
+// max_part_bytes > 0 caps the depth so that the partial tiles (the kx = 0
+// slice included) fit that many bytes; nz_main = 0 reports that even one
+// chunk does not fit.
+OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym, size_t max_part_bytes = 0) {
   OvlPlan pl;
   const int t128 = sym ? cdiv(M, 128) * (cdiv(M, 128) + 1) / 2 : cdiv(M, 128) * cdiv(N, 128);
   pl.WM = (M >= 256 || t128 >= 16) ? 4 : 2;
@@ -694,6 +697,17 @@ OvlPlan plan_ovl(int M, int N, int K, int kx0_end, bool sym) {
   pl.ntiles = sym ? Tm * (Tm + 1) / 2 : Tm * Tn;
   const int Kmain = K - kx0_end;
   int nz = ovl_depth(pl.ntiles, max(K, 1), pl.WM == 4 ? 2 : 4);
+  if (max_part_bytes > 0) {
+    const size_t per = (size_t)pl.ntiles * pl.BM * pl.BM * sizeof(double);
+    const long fit = (long)(max_part_bytes / per) - (kx0_end > 0 ? 1 : 0);
+    if (fit < 1) {
+      pl.nz_main = 0;
+      pl.kchunk = 0;
+      pl.nz = 0;
+      return pl;
+    }
+    if (nz > fit) nz = (int)fit;
+  }
   int kchunk = cdiv(cdiv(max(Kmain, 1), nz), BK) * BK;
   nz = max(1, cdiv(max(Kmain, 1), kchunk));
   pl.nz_main = nz;
@@ -766,6 +780,21 @@ int oa_launch_overlap_f64(hipStream_t s, int n, int K, const double *psi, long l
   const OvlPlan pl = plan_ovl(n, n, K, kx0_end, true);
   const size_t need = (size_t)pl.nz * pl.ntiles * pl.BM * pl.BM * sizeof(double);
   if (need > ws_bytes) return OA_ERR_CAPACITY;
+  OvlArgs a;
+  a.X = psi; a.ldx = ld; a.Y = psi; a.ldy = ld;
+  a.M = n; a.N = n; a.K = K;
+  a.kx0_end = kx0_end; a.nz_main = pl.nz_main; a.kchunk = pl.kchunk;
+  a.part = (double *)ws;
+  unsigned long long *md = reinterpret_cast<unsigned long long *>(maxdev);
+  if (pl.WM == 4) return launch_ovl<4, true>(s, pl, a, n, n, kx0_end, alpha, S, n, diag, md);
+  return launch_ovl<2, true>(s, pl, a, n, n, kx0_end, alpha, S, n, diag, md);
+}
+
+int oa_launch_overlap_f64_fit(hipStream_t s, int n, int K, const double *psi, long ld,
+                              int kx0_end, double alpha, double *S, double diag,
+                              double *maxdev, void *ws, size_t ws_bytes) {
+  const OvlPlan pl = plan_ovl(n, n, K, kx0_end, true, ws_bytes);
+  if (pl.nz_main < 1) return OA_ERR_CAPACITY;
   OvlArgs a;
   a.X = psi; a.ldx = ld; a.Y = psi; a.ldy = ld;
   a.M = n; a.N = n; a.K = K;

@@ -48,6 +48,16 @@ struct oa_context {
 
 int oa_ws_require(oa_context *c, size_t bytes);
 
+// Looks up device `device`'s copy of the no-alloc step's scratch array
+// (once per device; the device is current).
+int oa_noalloc_init(int device);
+
+// Synchronous S->T on the handle's stream with the buffers at ws
+// (oa_ns_ws_bytes(n)) and the device report rep; the host reads rep through
+// the handle's pinned mirror every two iterations.
+int oa_s_to_t_sync(oa_context *c, int n, const double *S, int nspin, double tol, int maxit,
+                   double *T, int *iters, double *err, void *ws, oa_ns_report *rep);
+
 // Makes the handle's device current for the scope of an API call and
 // restores the caller's device afterwards.
 struct DeviceGuard {
@@ -100,6 +110,13 @@ size_t oa_overlap_f64_ws_bytes(int n, int K);
 int oa_launch_overlap_f64(hipStream_t s, int n, int K, const double *psi,
                           long ld, int kx0_end, double alpha, double *S,
                           double diag, double *maxdev, void *ws, size_t ws_bytes);
+
+// The same product with the split depth capped so that its partial tiles fit
+// ws_bytes (OA_ERR_CAPACITY when one chunk does not); the sums differ from
+// oa_launch_overlap_f64's only by the chunking of the k sum.
+int oa_launch_overlap_f64_fit(hipStream_t s, int n, int K, const double *psi, long ld,
+                              int kx0_end, double alpha, double *S, double diag,
+                              double *maxdev, void *ws, size_t ws_bytes);
 
 // Rectangular split-K product C[M][N] = alpha * X[M][K] * Y[N][K]^T (Lambda).
 size_t oa_ntsplit_f64_ws_bytes(int M, int N, int K);
