@@ -753,19 +753,36 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
   // Wave-load i of a stage: operand o, slice p, quarter q (rows 32q..32q+31).
   // Lane L fills LDS slot q*64 + L of the panel, i.e. row r = (q*64+L)/2,
   // stored half hs = L%2, which holds global half hs ^ ((r >> 3) & 1).
+  // Load j of the wave's share of a stage: operand o, slice p and quarter q
+  // are wave-uniform, so the global address splits into a scalar part (the
+  // operand, slice and tile rows, plus k-block times its pitch) and a
+  // per-lane offset (row r of the quarter, 16-byte half h) fixed for the
+  // tile; each stage then adds one scalar product per load.
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const char *sbase[PERW];
+  size_t spitch[PERW];
+  unsigned voff[PERW];
+  int loff[PERW];
+#pragma unroll
+  for (int j = 0; j < PERW; j++) {
+    const int i = wu + NW * j;
+    const int o = i / (NL * NQ), p = (i / NQ) % NL, q = i % NQ;
+    const int pos = q * 64 + lane, r = pos >> 1, h = (pos & 1) ^ ((r >> 3) & 1);
+    sbase[j] = (o == 0) ? Xg + ((size_t)p * sx + (size_t)m0 * SK) * ES
+                        : Yg + ((size_t)p * sy + (size_t)n0 * SK) * ES;
+    spitch[j] = (size_t)((o == 0) ? ldx : ldy) * SK * ES;
+    voff[j] = (unsigned)(r * SK * ES + h * 16);
+    loff[j] = o * NL * PANEL + p * PANEL + q * 1024;
+  }
   auto issue = [&](int st) {
-    const int kblk = DIAG_LDS ? (st + rot) % nst : (ka + st * SK) / SK;
+    int kblk = DIAG_LDS ? st + rot : (ka + st * SK) / SK;
+    if (DIAG_LDS && kblk >= nst) kblk -= nst;
     char *buf = smem + (st % NBUF) * STAGE;
 #pragma unroll
     for (int j = 0; j < PERW; j++) {
-      const int i = w + NW * j;
-      const int o = i / (NL * NQ), p = (i / NQ) % NL, q = i % NQ;
-      const int pos = q * 64 + lane, r = pos >> 1, h = (pos & 1) ^ ((r >> 3) & 1);
-      const char *g = (o == 0)
-          ? Xg + ((size_t)p * sx + ((size_t)kblk * ldx + m0 + r) * SK) * ES + h * 16
-          : Yg + ((size_t)p * sy + ((size_t)kblk * ldy + n0 + r) * SK) * ES + h * 16;
-      char *l = buf + o * NL * PANEL + p * PANEL + q * 1024;
-      __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)l, 16, 0, 0);
+      const char *g = sbase[j] + (size_t)kblk * spitch[j] + voff[j];
+      __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)(buf + loff[j]),
+                                       16, 0, 0);
     }
   };
 #pragma unroll
@@ -828,20 +845,15 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
       return val;
     };
     if (MODE != 1) {
+      // the chunk's scale, read once (the partial-tile stores could alias it)
+      const int ex = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
+      const int sc = (MODE == 0) ? 2 * (ex - 7)
+                                 : ex + chunk_exp(__longlong_as_double((long long)a.colmax[z])) - 14;
+      double *pt = a.part + ((size_t)z * a.ntiles + tile) * (TILE * TILE) + ln;
 #pragma unroll
       for (int v = 0; v < 16; v++) {
-        const double val = level_sum(v);
         const int lm = xr + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (MODE == 0) {
-          const int e = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
-          a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
-              ldexp(val, 2 * (e - 7));
-        } else {
-          const int ex = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
-          const int ey = chunk_exp(__longlong_as_double((long long)a.colmax[z]));
-          a.part[((size_t)z * a.ntiles + tile) * (TILE * TILE) + lm * TILE + ln] =
-              ldexp(val, ex + ey - 14);
-        }
+        pt[lm * TILE] = ldexp(level_sum(v), sc);
       }
       continue;
     }
@@ -865,22 +877,50 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
       // stage nst - ND + xr/SK, and the 4 rows are bytes b0..b0+3 of LDS row
       // ln (16-byte halves swapped on odd groups of 8 rows, as stored)
       const int b0 = 8 * g + 4 * (lane >> 5);
+      if constexpr (DT == DT_I8) {
+        // the four rows' digits of slice p sit in one dword (byte r = row
+        // mg + r); the leading min(S, 4) digits combine exactly in int32
+        // (|sum| <= 127 (254^3 + 254^2 + 254 + 1) < 2^31), absent digits
+        // (p >= SD) as zeros, and the rest (S > 4) add in FP64
+        constexpr int PI = S < 4 ? S : 4;
+        int wq[S];
 #pragma unroll
-      for (int p = S - 1; p >= 0; p--) {
-        if (p >= SD) continue;
-        double q4[4];
-        if (DIAG_LDS && p < NL) {
-          const char *yb = smem + ((nst - ND + xr / SK) % NBUF) * STAGE + NL * PANEL + p * PANEL;
-          const int wv = *reinterpret_cast<const int *>(
-              yb + ln * 32 + (((b0 >> 4) ^ ((ln >> 3) & 1)) << 4) + (b0 & 15));
+        for (int p = 0; p < S; p++) {
+          wq[p] = 0;
+          if (p >= SD) continue;
+          if (DIAG_LDS && p < NL) {
+            const char *yb = smem + ((nst - ND + xr / SK) % NBUF) * STAGE + NL * PANEL + p * PANEL;
+            wq[p] = *reinterpret_cast<const int *>(
+                yb + ln * 32 + (((b0 >> 4) ^ ((ln >> 3) & 1)) << 4) + (b0 & 15));
+          } else {
+            wq[p] = *reinterpret_cast<const int *>(
+                reinterpret_cast<const int8_t *>(a.DB) + (size_t)p * a.dbs +
+                kb_off<DT>(min(nn, a.N - 1), mg, (size_t)a.Kp));
+          }
+        }
+        // explicit fma pins the rounding, so every instantiation (uniform or
+        // per chunk slice counts) gives the same bits
 #pragma unroll
-          for (int r = 0; r < 4; r++) q4[r] = (double)(int8_t)((wv >> (8 * r)) & 0xff);
-        } else {
+        for (int r = 0; r < 4; r++) {
+          int iv = 0;
+#pragma unroll
+          for (int p = 0; p < PI; p++) iv = iv * 254 + (int)(int8_t)((wq[p] >> (8 * r)) & 0xff);
+          double t = 0.0;
+#pragma unroll
+          for (int p = S - 1; p >= PI; p--)
+            t = fma((double)(int8_t)((wq[p] >> (8 * r)) & 0xff), rinv(p), t);
+          pd[r] = fma((double)iv, rinv(PI - 1), t);
+        }
+      } else {
+#pragma unroll
+        for (int p = S - 1; p >= 0; p--) {
+          if (p >= SD) continue;
+          double q4[4];
           load_digit_quad<DT>(a.DB, (size_t)p * a.dbs + kb_off<DT>(min(nn, a.N - 1), mg,
                                                                      (size_t)a.Kp), q4);
-        }
 #pragma unroll
-        for (int r = 0; r < 4; r++) pd[r] += q4[r] * rinv(p);
+          for (int r = 0; r < 4; r++) pd[r] += q4[r] * rinv(p);
+        }
       }
       auto value = [&](int r) {
         return al * (dv[r] * ldexp(pd[r], ez - 7) +
