@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--maxit", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="large workload: skip the batched (64 uneven instances) sub-object")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="large workload: skip the serving sub-object")
     ap.add_argument("--serve-tune", action="store_true",
                     help="serving: tune the tile table before the run on a synthetic mix "
                          "(helps at saturation, hurts batches of one or two sessions)")
@@ -197,9 +201,10 @@ def batched_instances(count=64, seed=0):
             for i, (n, p, e) in enumerate(zip(ns, npws, epss))]
 
 
-def run_batched(args):
-    """Config of many independent instances: the grouped batch (tuned tiles,
-    hipGraph replay) against a loop of single-instance steps."""
+def batched_results(args, modes, steps, warmup):
+    """The 64 uneven instances: per mode, the grouped batch (tuned tiles,
+    hipGraph replay) timed over `steps` launches, against a loop of
+    single-instance steps; returns (results per mode, headline mode)."""
     import torch
 
     from openatom_amd.ortho import Batch, Handle
@@ -209,7 +214,8 @@ def run_batched(args):
     h = Handle(0)
     stream = torch.cuda.current_stream()
     h.use_stream(stream)
-    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
+    # algorithmic FP64 work per batch step: overlap (upper triangle), T*psi
+    # and 6 n^3 per S->T iteration (iterations from the reports)
     flops = sum(2.0 * p.shape[0] * (p.shape[0] + 1) * (p.shape[1] // 2)
                 + 4.0 * p.shape[0] ** 2 * (p.shape[1] // 2) for p in psis)
     res = {}
@@ -219,11 +225,11 @@ def run_batched(args):
         t0 = time.perf_counter()
         b.tune(reps=2)
         tune_s = time.perf_counter() - t0
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             b.launch()
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
+              for _ in range(steps)]
         t0 = time.perf_counter()
         for e0, e1 in ev:
             e0.record(stream)
@@ -233,6 +239,7 @@ def run_batched(args):
         wall = time.perf_counter() - t0
         ms = sorted(e0.elapsed_time(e1) for e0, e1 in ev)
         reps = b.reports()
+        st_flops = sum(6.0 * p.shape[0] ** 3 * r[0] for p, r in zip(psis, reps))
         eye_res = 0.0
         for i, o in enumerate(b.outs):
             S2 = h.overlap_sym(o, nkx0=1, alpha=2.0, prec="fp64")
@@ -252,19 +259,22 @@ def run_batched(args):
         loop()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             loop()
         torch.cuda.synchronize()
         wall_loop = time.perf_counter() - t1
         same = all(torch.equal(outs[i], b.outs[i]) for i in range(len(psis)))
+        fp64_eq = (flops + st_flops) * steps / wall / 1e12
         res[prec] = {
-            "instance_steps_per_s": len(psis) * args.steps / wall,
-            "batch_ms": 1e3 * wall / args.steps,
+            "instance_steps_per_s": len(psis) * steps / wall,
+            "batch_ms": 1e3 * wall / steps,
             "p50_ms": ms[len(ms) // 2],
             "p99_ms": ms[min(len(ms) - 1, int(0.99 * len(ms)))],
-            "loop_instance_steps_per_s": len(psis) * args.steps / wall_loop,
+            "loop_instance_steps_per_s": len(psis) * steps / wall_loop,
             "speedup_vs_loop": wall_loop / wall,
-            "algorithmic_tflops": flops * args.steps / wall / 1e12,
+            "algorithmic_tflops": flops * steps / wall / 1e12,
+            "fp64_equivalent_tflops": fp64_eq,
+            "fp64_equivalent_frac_of_fp64_peak": fp64_eq / PEAK_FP64_TFLOPS,
             "max_iterations": max(r[0] for r in reps),
             "all_converged": all(r[2] == 0 for r in reps),
             "max_residual": eye_res,
@@ -275,6 +285,16 @@ def run_batched(args):
         b.close()
     head_k = max((k for k in res if res[k]["max_residual"] <= RESID_TARGET),
                  key=lambda k: res[k]["instance_steps_per_s"], default=modes[0])
+    del psis
+    torch.cuda.empty_cache()
+    return res, head_k
+
+
+def run_batched(args):
+    """Config of many independent instances: the grouped batch (tuned tiles,
+    hipGraph replay) against a loop of single-instance steps."""
+    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
+    res, head_k = batched_results(args, modes, args.steps, args.warmup)
     head = res[head_k]
     line = {
         "metric": "instance ortho steps/s, 64 uneven instances per grouped step",
@@ -293,10 +313,11 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
-def run_serving(args):
-    """Continuous-batching loop: Poisson sessions (n in {128,256,512,1024}
-    with weights .4/.3/.2/.1, npw = 64 n) that stay resident for
-    --serve-steps MD steps, one request per step; admission by HBM footprint."""
+def serving_results(args, prec, rates, duration):
+    """The continuous-batching loop at each offered rate: Poisson sessions (n
+    in {128,256,512,1024} with weights .4/.3/.2/.1, npw = 64 n) resident for
+    --serve-steps MD steps, one request per step; returns the sweep and the
+    HBM budget."""
     import torch
 
     from openatom_amd.ortho import Handle
@@ -304,12 +325,11 @@ def run_serving(args):
     torch.cuda.set_device(0)
     h = Handle(0)
     h.use_stream(torch.cuda.current_stream())
-    prec = [m.strip() for m in args.modes.split(",") if m.strip()][-1]
     free, total = torch.cuda.mem_get_info()
     budget = int(min(0.85 * free, 0.9 * 288e9))
     sweep = []
-    for rate in [float(r) for r in args.rates.split(",") if r.strip()]:
-        specs = poisson_sessions(rate, args.duration, steps=args.serve_steps, seed=0)
+    for rate in rates:
+        specs = poisson_sessions(rate, duration, steps=args.serve_steps, seed=0)
         # the batch workspace of all admitted sessions: at most ~60% of their
         # footprints; sized once so re-plans never allocate
         ex = GpuExecutor(h, prec=prec, tol=args.tol, maxit=args.maxit,
@@ -317,7 +337,7 @@ def run_serving(args):
         # optionally, the tile table is tuned once, before the clock starts, on
         # a synthetic mix of the same size distribution
         tune_s = ex.tune() if args.serve_tune else 0.0
-        r = serve(specs, ex, budget, max_wall_s=args.duration + 120.0)
+        r = serve(specs, ex, budget, max_wall_s=duration + 120.0)
         r.update({"replans": ex.replans, "plan_s": ex.t_plan, "exec_s": ex.t_exec,
                   "tune_s_before_run": tune_s,
                   "tile_table": ex.batch.table() if args.serve_tune else None})
@@ -328,6 +348,14 @@ def run_serving(args):
         print(f"# rate {rate:g}: {r['throughput_req_s']:.1f} req/s, p50 "
               f"{r['latency_p50_ms']} ms, p99 {r['latency_p99_ms']} ms, mean batch "
               f"{r['mean_batch']:.1f}", file=sys.stderr, flush=True)
+    return sweep, budget
+
+
+def run_serving(args):
+    """Continuous-batching loop over the --rates sweep (one JSON line)."""
+    prec = [m.strip() for m in args.modes.split(",") if m.strip()][-1]
+    rates = [float(r) for r in args.rates.split(",") if r.strip()]
+    sweep, budget = serving_results(args, prec, rates, args.duration)
     top = sweep[-1]
     line = {
         "metric": "serving: completed ortho-step requests/s at the highest offered rate; "
@@ -742,6 +770,37 @@ def main():
             "algorithmic_per_launch": alg,
         },
     }
+    if args.workload == "large" and not (args.no_batched and args.no_serving):
+        del psi
+        torch.cuda.empty_cache()
+    split_mode = head["prec"] if head["prec"] != "fp64" else "int8x6a"
+    if args.workload == "large" and not args.no_batched:
+        bmodes = [split_mode, "fp64"]
+        bres, bhead = batched_results(args, bmodes, steps=10, warmup=2)
+        keep = ("instance_steps_per_s", "batch_ms", "p50_ms", "p99_ms", "max_residual",
+                "all_converged", "max_iterations", "fp64_equivalent_tflops",
+                "fp64_equivalent_frac_of_fp64_peak", "speedup_vs_loop",
+                "loop_instance_steps_per_s", "identical_to_single_instance", "tune_s")
+        line["batched"] = {
+            "workload": BATCHED, "steps": 10, "warmup": 2, "headline_mode": bhead,
+            "unit": "instance steps/s",
+            "modes": {k: {f: v[f] for f in keep} for k, v in bres.items()},
+        }
+    if args.workload == "large" and not args.no_serving:
+        rates, duration = [1000.0, 2000.0], 2.5
+        sweep, budget = serving_results(args, split_mode, rates, duration)
+        keep = ("offered_req_s", "throughput_req_s", "latency_p50_ms", "latency_p99_ms",
+                "admission_wait_p50_ms", "admission_wait_p99_ms", "mean_batch", "sessions",
+                "sessions_finished", "requests_completed", "elapsed_s", "replans",
+                "latency_by_n_ms")
+        line["serving"] = {
+            "workload": (f"Poisson sessions, n in {{128,256,512,1024}} (p = .4/.3/.2/.1), "
+                         f"npw = 64 n, {args.serve_steps} MD steps per session, "
+                         f"{duration:g} s of arrivals per rate"),
+            "prec": split_mode, "budget_bytes": budget,
+            "note": "completed requests/s over the run's wall time, drain included",
+            "rates": [{f: r.get(f) for f in keep} for r in sweep],
+        }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(n, npw, iters, threads)

@@ -135,6 +135,7 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
     sched = Scheduler(budget_bytes, executor.footprint)
     pending = deque(sorted(specs, key=lambda s: s.arrival))
     waits, lat = [], []
+    lat_by_n: dict[int, list] = {}
     finished, iterations, batch_sizes = [], 0, []
     t_step = t_advance = 0.0  # loop time in batch steps / client MD steps
     while pending or sched.waiting or sched.active:
@@ -169,6 +170,7 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
         for s in batch:
             s.latencies.append(t_done - s.issued_at)
             lat.append(t_done - s.issued_at)
+            lat_by_n.setdefault(s.spec.n, []).append(t_done - s.issued_at)
             s.done += 1
             if s.done >= s.spec.steps:
                 executor.close(s)
@@ -195,6 +197,11 @@ def serve(specs: list[SessionSpec], executor, budget_bytes: int, clock=None,
         "throughput_req_s": len(lat) / elapsed if elapsed > 0 else 0.0,
         "latency_p50_ms": None if not lat else 1e3 * pct(lat, 50),
         "latency_p99_ms": None if not lat else 1e3 * pct(lat, 99),
+        # per session size: the mix puts n=1024 steps (512x the work of
+        # n=128) in the upper percentiles of the pooled latencies
+        "latency_by_n_ms": {str(k): {"requests": len(v), "p50": 1e3 * pct(v, 50),
+                                     "p99": 1e3 * pct(v, 99)}
+                            for k, v in sorted(lat_by_n.items())},
         "admission_wait_p50_ms": None if not waits else 1e3 * pct(waits, 50),
         "admission_wait_p99_ms": None if not waits else 1e3 * pct(waits, 99),
         "iterations": iterations,
