@@ -169,6 +169,58 @@ __device__ __forceinline__ void kloop(const double *X, long ldx, int m0, int mli
   }
 }
 
+// kloop with the loads issued two stages ahead in two register sets, for the
+// 64 x 64 tiles of the small n x n products (S->T, small back-multiplies),
+// where one stage of MFMA work is shorter than a global load's latency.  The
+// stages and their products run in kloop's order, so the sums are
+// bit-identical.  Register set s % 2 holds stage s + 1 when stage s starts.
+template <int WM, bool XKC, bool YKC>
+__device__ __forceinline__ void kloop_pf2(const double *X, long ldx, int m0, int mlim,
+                                          const double *Y, long ldy, int n0, int nlim, int ka,
+                                          int kb, bool fast, double *smem, d4 (&acc)[WM][WM]) {
+  constexpr int BM = 32 * WM;
+  constexpr int PANEL = BM * LS;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
+  PanelLoader<BM, XKC> lx0, lx1;
+  PanelLoader<BM, YKC> ly0, ly1;
+  lx0.init(X, ldx, tid);
+  lx1.init(X, ldx, tid);
+  ly0.init(Y, ldy, tid);
+  ly1.init(Y, ldy, tid);
+  const int nst = (kb - ka + BK - 1) / BK;
+  if (nst <= 0) return;
+  auto fetch = [&](PanelLoader<BM, XKC> &lx, PanelLoader<BM, YKC> &ly, int k0) {
+    if (fast && k0 + BK <= kb) {
+      lx.load_fast(m0, k0);
+      ly.load_fast(n0, k0);
+    } else {
+      lx.load_masked(m0, mlim, k0, kb);
+      ly.load_masked(n0, nlim, k0, kb);
+    }
+  };
+  fetch(lx0, ly0, ka);
+  lx0.store(smem);
+  ly0.store(smem + 2 * PANEL);
+  if (nst > 1) fetch(lx0, ly0, ka + BK);
+  if (nst > 2) fetch(lx1, ly1, ka + 2 * BK);
+  __syncthreads();
+  auto step = [&](int s, PanelLoader<BM, XKC> &lx, PanelLoader<BM, YKC> &ly) {
+    const int cur = (s & 1) * PANEL, nxt = PANEL - cur;
+    mma_stage<WM>(smem + cur, smem + 2 * PANEL + cur, xrow0, yrow0, lane, acc);
+    if (s + 1 < nst) {
+      lx.store(smem + nxt);
+      ly.store(smem + 2 * PANEL + nxt);
+      if (s + 3 < nst) fetch(lx, ly, ka + (s + 3) * BK);
+    }
+    __syncthreads();
+  };
+  for (int s = 0; s < nst; s += 2) {
+    step(s, lx0, ly0);
+    if (s + 1 < nst) step(s + 1, lx1, ly1);
+  }
+}
+
 __device__ __forceinline__ bool aligned16(const void *p) {
   return ((uintptr_t)p & 15) == 0;
 }
@@ -224,7 +276,11 @@ __device__ __forceinline__ void gemm_body(const ProbRegs &P, int logical, double
     for (int j = 0; j < WM; j++) acc[i][j] = d4{0, 0, 0, 0};
   const bool fast = m0 + BM <= P.M && n0 + BM <= P.N && aligned16(P.X) && aligned16(P.Y) &&
                     (P.ldx % 2) == 0 && (P.ldy % 2) == 0;
-  kloop<WM, !TX, TY, false>(P.X, P.ldx, m0, P.M, P.Y, P.ldy, n0, P.N, 0, P.K, fast, smem, acc);
+  if constexpr (WM == 2)
+    kloop_pf2<WM, !TX, TY>(P.X, P.ldx, m0, P.M, P.Y, P.ldy, n0, P.N, 0, P.K, fast, smem, acc);
+  else
+    kloop<WM, !TX, TY, false>(P.X, P.ldx, m0, P.M, P.Y, P.ldy, n0, P.N, 0, P.K, fast, smem,
+                              acc);
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int xrow0 = (w >> 1) * 16 * WM, yrow0 = (w & 1) * 16 * WM;
