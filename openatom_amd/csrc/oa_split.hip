@@ -587,11 +587,12 @@ __device__ __forceinline__ void upper_tile_d(int tile, int T, int &bi, int &bj) 
 // Panels hold TILE rows x 32 bytes per slice with the two 16-byte halves of
 // row r swapped when bit 3 of r is set, which spreads any 16 consecutive
 // lanes' 16-byte fragment reads over all 64 LDS banks.
-template <int DT, int NL, int WI, int WJ, int TILE>
+template <int DT, int NL, int WI, int WJ, int TILE, int TN = TILE>
 __device__ __forceinline__ void digit_stage(const char *Xs, const char *Ys, int xrow, int yrow,
                                             int lane,
                                             typename DTraits<DT>::acc_t (&acc)[NL][WI][WJ]) {
-  constexpr int PANEL = TILE * 32;
+  constexpr int PANEL = TILE * 32;   // X slice panel (TILE rows of 32 bytes)
+  constexpr int YPANEL = TN * 32;    // Y slice panel (TN rows)
   const int r = lane & 31;
   const int boff = r * 32 + 16 * ((lane >> 5) ^ ((r >> 3) & 1));
   i4v a[NL][WI];
@@ -605,7 +606,7 @@ __device__ __forceinline__ void digit_stage(const char *Xs, const char *Ys, int 
     i4v b[WJ];
 #pragma unroll
     for (int j = 0; j < WJ; j++)
-      b[j] = *reinterpret_cast<const i4v *>(Ys + q * PANEL + (yrow + j * 32) * 32 + boff);
+      b[j] = *reinterpret_cast<const i4v *>(Ys + q * YPANEL + (yrow + j * 32) * 32 + boff);
 #pragma unroll
     for (int p = 0; p + q < NL; p++)
 #pragma unroll
@@ -692,33 +693,43 @@ struct DigitCfg {
 // item from the chunk's slice count.  The X and Y panels of NBUF stages
 // (slots sized for NL slices) rotate through LDS, filled by direct-to-LDS
 // loads (global_load_lds_dwordx4) NBUF-1 stages ahead.
-template <int DT, int S, int MODE, int NLB, int TILE, int NLR>
+//   TN: output columns of the tile (TILE, or 2 TILE for the wide back-multiply
+//       tile of 64 x 64 wave tiles, used at two levels or fewer).
+template <int DT, int S, int MODE, int NLB, int TILE, int NLR, int TN = TILE>
 __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int bj, int z,
                                            int tile, int SD, char *smem) {
   typedef typename DTraits<DT>::acc_t ACC;
   typedef DigitCfg<DT, S, MODE, NLB, TILE> C;
+  constexpr bool WIDE = TN != TILE;
+  static_assert(!WIDE || (MODE == 1 && DT == DT_I8 && TN == 2 * TILE && NLR <= 3),
+                "the wide tile is the int8 back-multiply's, at three levels or fewer");
   constexpr int NL = NLR;
   static_assert(NLR >= 1 && NLR <= C::NL, "levels within the ring's slots");
   constexpr int ES = DTraits<DT>::ES;
   constexpr int SK = 32 / ES;             // contraction elements per stage
-  constexpr int PANEL = C::PANEL;
-  constexpr int STAGE = C::STAGE;
-  constexpr int NBUF = C::NBUF;
+  constexpr int PANEL = C::PANEL;         // X slice panel: TILE rows x 32 bytes
+  constexpr int YPANEL = TN * 32;         // Y slice panel: TN rows x 32 bytes
+  // ring slots: the configuration's (sized for its largest level count), or
+  // for the wide tile as many of its own stages as the same LDS holds (<= 5)
+  constexpr int STAGE = WIDE ? NL * (PANEL + YPANEL) : C::STAGE;
+  constexpr int NBUF = WIDE ? (C::SMEM / STAGE >= 5 ? 5 : C::SMEM / STAGE) : C::NBUF;
+  static_assert(NBUF >= 3, "ring of at least three stages");
   constexpr int NW = C::NW;
-  constexpr int WI = C::WI;
-  constexpr int WJ = C::WJ;
-  constexpr int WC = TILE / (32 * WJ);    // waves per tile row
+  constexpr int WI = WIDE ? 2 : C::WI;
+  constexpr int WJ = WIDE ? 2 : C::WJ;
+  constexpr int WC = TN / (32 * WJ);      // waves per tile row
   static_assert(NW == (TILE / (32 * WI)) * WC, "waves must cover the tile");
-  constexpr int NQ = TILE / 32;           // 1 KB wave-loads per slice panel
-  constexpr int NI = 2 * NL * NQ;         // wave-loads per stage
-  constexpr int PERW = NI / NW;           // per wave
-  static_assert(NI % NW == 0, "stage loads must split evenly over the waves");
+  constexpr int NQ = TILE / 32;           // 1 KB wave-loads per X slice panel
+  constexpr int NQY = TN / 32;            // per Y slice panel
+  constexpr int NI = NL * (NQ + NQY);     // wave-loads per stage
+  constexpr int PERW = (NI + NW - 1) / NW;  // per wave (waves w >= NHI issue PERW - 1)
+  constexpr int NHI = NI % NW;
   // fp32 accumulation is exact only below 2^24 (level sums < 6 * 127^2 =
   // 96774 per k for S <= 6): promote every PROMOTE stages (128 k) into one
   // FP64 sum per output
   constexpr int PROMOTE = (DT == DT_I8) ? (1 << 30) : 128 / SK;
 
-  const int m0 = bi * TILE, n0 = bj * TILE;
+  const int m0 = bi * TILE, n0 = bj * TN;
   const int ka = (MODE != 1) ? z * a.kc : 0;
   const int kb = (MODE != 1) ? min(a.kfull, ka + a.kc) : a.kfull;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -759,20 +770,23 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
   // per-lane offset (row r of the quarter, 16-byte half h) fixed for the
   // tile; each stage then adds one scalar product per load.
   const int wu = __builtin_amdgcn_readfirstlane(w);
+  const bool whi = NHI == 0 || wu < NHI;  // this wave issues PERW loads per stage
   const char *sbase[PERW];
   size_t spitch[PERW];
   unsigned voff[PERW];
   int loff[PERW];
 #pragma unroll
   for (int j = 0; j < PERW; j++) {
-    const int i = wu + NW * j;
-    const int o = i / (NL * NQ), p = (i / NQ) % NL, q = i % NQ;
+    const int i = min(wu + NW * j, NI - 1);   // loads past NI are not issued
+    const bool x = i < NL * NQ;               // loads 0 .. NL*NQ-1: X, then Y
+    const int iy = i - NL * NQ;
+    const int p = x ? i / NQ : iy / NQY, q = x ? i % NQ : iy % NQY;
     const int pos = q * 64 + lane, r = pos >> 1, h = (pos & 1) ^ ((r >> 3) & 1);
-    sbase[j] = (o == 0) ? Xg + ((size_t)p * sx + (size_t)m0 * SK) * ES
-                        : Yg + ((size_t)p * sy + (size_t)n0 * SK) * ES;
-    spitch[j] = (size_t)((o == 0) ? ldx : ldy) * SK * ES;
+    sbase[j] = x ? Xg + ((size_t)p * sx + (size_t)m0 * SK) * ES
+                 : Yg + ((size_t)p * sy + (size_t)n0 * SK) * ES;
+    spitch[j] = (size_t)(x ? ldx : ldy) * SK * ES;
     voff[j] = (unsigned)(r * SK * ES + h * 16);
-    loff[j] = o * NL * PANEL + p * PANEL + q * 1024;
+    loff[j] = x ? p * PANEL + q * 1024 : NL * PANEL + p * YPANEL + q * 1024;
   }
   auto issue = [&](int st) {
     int kblk = DIAG_LDS ? st + rot : (ka + st * SK) / SK;
@@ -780,9 +794,25 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
     char *buf = smem + (st % NBUF) * STAGE;
 #pragma unroll
     for (int j = 0; j < PERW; j++) {
+      if (NHI != 0 && j == PERW - 1 && !whi) break;
       const char *g = sbase[j] + (size_t)kblk * spitch[j] + voff[j];
       __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void *)(buf + loff[j]),
                                        16, 0, 0);
+    }
+  };
+  // waits until at most c stages of this wave's direct-to-LDS loads are in
+  // flight (the loop issues no other vector memory operation)
+  auto wait_stages = [&](int c) {
+    if (NHI == 0 || whi) {
+      if (NBUF >= 5 && c >= 3) wait_vm<3 * PERW>();
+      else if (NBUF >= 4 && c >= 2) wait_vm<2 * PERW>();
+      else if (c >= 1) wait_vm<PERW>();
+      else wait_vm<0>();
+    } else {
+      if (NBUF >= 5 && c >= 3) wait_vm<3 * (PERW - 1)>();
+      else if (NBUF >= 4 && c >= 2) wait_vm<2 * (PERW - 1)>();
+      else if (c >= 1) wait_vm<PERW - 1>();
+      else wait_vm<0>();
     }
   };
 #pragma unroll
@@ -790,15 +820,11 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
     if (st < nst) issue(st);
   for (int s = 0; s < nst; s++) {
     // stages s+1 .. min(nst-1, s+NBUF-2) may stay in flight
-    const int ahead = min(nst - 1 - s, NBUF - 2);
-    if (NBUF >= 5 && ahead >= 3) wait_vm<3 * PERW>();
-    else if (NBUF >= 4 && ahead >= 2) wait_vm<2 * PERW>();
-    else if (ahead >= 1) wait_vm<PERW>();
-    else wait_vm<0>();
+    wait_stages(min(nst - 1 - s, NBUF - 2));
     ring_barrier();
     if (s + NBUF - 1 < nst) issue(s + NBUF - 1);
     const char *cur = smem + (s % NBUF) * STAGE;
-    digit_stage<DT, NL, WI, WJ, TILE>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
+    digit_stage<DT, NL, WI, WJ, TILE, TN>(cur, cur + NL * PANEL, xrow, yrow, lane, acc);
     if (DT != DT_I8 && ((s + 1) % PROMOTE == 0 || s + 1 == nst)) {
 #pragma unroll
       for (int i = 0; i < WI; i++)
@@ -820,13 +846,31 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
     }
   }
 
+  // int8: the exact level sums fold into one FP64 value per output right
+  // after the loop (smallest level first, explicit fma so every
+  // instantiation rounds alike), which frees the NL int32 accumulators before
+  // the epilogue's temporaries are live
+  if constexpr (DT == DT_I8) {
+#pragma unroll
+    for (int i = 0; i < WI; i++)
+#pragma unroll
+      for (int j = 0; j < WJ; j++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+          double val = 0.0;
+#pragma unroll
+          for (int l = NL - 1; l >= 0; l--) val = fma((double)acc[l][i][j][v], rinv(l), val);
+          facc[i][j][v] = val;
+        }
+  }
+
   // epilogue: combine levels in FP64.  MODE 1 works on groups of 4 rows per
   // lane: it loads T's per-row exponent and FP64 diagonal together with the
   // psi~ digits of those rows, and tests beta and the tile edge outside the
   // element loop, so an interior tile pays one memory round trip per group
   // (a per-element bounds test kept the compiler from hoisting the loads: one
   // round trip per output).
-  const bool edge = (m0 + TILE > a.M) || (n0 + TILE > a.N);
+  const bool edge = (m0 + TILE > a.M) || (n0 + TN > a.N);
 #pragma unroll
   for (int i = 0; i < WI; i++)
 #pragma unroll
@@ -834,16 +878,7 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
     const int ln = yrow + j * 32 + (lane & 31);
     const int nn = n0 + ln;
     const int xr = xrow + i * 32;
-    auto level_sum = [&](int v) {
-      double val = 0.0;
-      if (DT == DT_I8) {
-#pragma unroll
-        for (int l = NL - 1; l >= 0; l--) val += (double)acc[l][i][j][v] * rinv(l);
-      } else {
-        val = facc[i][j][v];
-      }
-      return val;
-    };
+    auto level_sum = [&](int v) { return facc[i][j][v]; };
     if (MODE != 1) {
       // the chunk's scale, read once (the partial-tile stores could alias it)
       const int ex = chunk_exp(__longlong_as_double((long long)a.chunk_max[z]));
@@ -857,8 +892,9 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
       }
       continue;
     }
-    const int ez =
-        chunk_exp(__longlong_as_double((long long)a.colmax[min(nn, a.N - 1) / a.kchunk]));
+    // a tile's columns lie in one chunk (kchunk is a multiple of 128 and a
+    // wide tile pairs two tiles of one chunk): one exponent per tile
+    const int ez = chunk_exp(__longlong_as_double((long long)a.colmax[n0 / a.kchunk]));
     const double al = (nn < a.cw) ? a.alpha_w : a.alpha;
 #pragma unroll
     for (int g = 0; g < 4; g++) {
@@ -889,7 +925,7 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
           wq[p] = 0;
           if (p >= SD) continue;
           if (DIAG_LDS && p < NL) {
-            const char *yb = smem + ((nst - ND + xr / SK) % NBUF) * STAGE + NL * PANEL + p * PANEL;
+            const char *yb = smem + ((nst - ND + xr / SK) % NBUF) * STAGE + NL * PANEL + p * YPANEL;
             wq[p] = *reinterpret_cast<const int *>(
                 yb + ln * 32 + (((b0 >> 4) ^ ((ln >> 3) & 1)) << 4) + (b0 & 15));
           } else {
@@ -1012,7 +1048,24 @@ __device__ __forceinline__ void digit_body(const oa_digit_args &a, int logical, 
         });
       } else {
         constexpr int LO = (NLB - (S - ADA_SMIN) > 1) ? NLB - (S - ADA_SMIN) : 1;
-        level_switch<LO, NLB>(NLB - (S - sz), [&](auto c) {
+        const int nlr = NLB - (S - sz);
+        if constexpr (TILE == 128 && LO <= 2) {
+          // chunks of at most two levels run 128 x 256 tiles: the even
+          // column tile of each pair takes both, the odd one returns (an
+          // even cpt keeps a pair inside one chunk).  At three levels the
+          // 64 x 64 wave tiles' accumulators leave the epilogue too few
+          // registers and the wide tile measured slower (4.54 -> 4.81 ms).
+          const int cpt = a.kchunk / TILE;
+          if (nlr <= 2 && (cpt & 1) == 0 && (bj | 1) < cdv(a.N, TILE)) {
+            if (bj & 1) return;
+            level_switch<LO, (NLB < 2 ? NLB : 2)>(nlr, [&](auto c) {
+              digit_core<DT, S, MODE, NLB, TILE, decltype(c)::value, 2 * TILE>(
+                  a, bi, bj >> 1, z, tile, sz, smem);
+            });
+            return;
+          }
+        }
+        level_switch<LO, NLB>(nlr, [&](auto c) {
           digit_core<DT, S, MODE, NLB, TILE, decltype(c)::value>(a, bi, bj, z, tile, sz, smem);
         });
       }
