@@ -298,7 +298,9 @@ __device__ __forceinline__ void gemm_body(const ProbRegs &P, int logical, double
         for (int j = 0; j < WM; j++) {
           const int n = n0 + yrow0 + j * 16 + r;
           const double al = (n < P.cw) ? P.alpha_w : alpha;
-          if (n < P.N) crow[n] = al * acc[i][j][v] + ((m == n) ? dadd : 0.0);
+          // explicit fma: single and grouped launches inline this body and
+          // must round alike
+          if (n < P.N) crow[n] = fma(al, acc[i][j][v], (m == n) ? dadd : 0.0);
         }
       }
   } else {
@@ -314,7 +316,7 @@ __device__ __forceinline__ void gemm_body(const ProbRegs &P, int logical, double
           const int n = n0 + yrow0 + j * 16 + r;
           const double al = (n < P.cw) ? P.alpha_w : alpha;
           if (n < P.N)
-            crow[n] = al * acc[i][j][v] + beta * crow[n] + ((m == n) ? dadd : 0.0);
+            crow[n] = fma(al, acc[i][j][v], fma(beta, crow[n], (m == n) ? dadd : 0.0));
         }
       }
   }

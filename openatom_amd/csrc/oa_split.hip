@@ -193,11 +193,17 @@ __host__ __device__ __forceinline__ double rinv(int l) {
 }
 
 // Split one value into S digits given its exponent.
+// Only the leading nd digits are formed (the rest are left at 0): an
+// adaptive split keeps s_z <= S digits of chunk z.
 template <int S>
-__device__ __forceinline__ void split_value(double x, int e, double (&d)[S]) {
+__device__ __forceinline__ void split_value(double x, int e, double (&d)[S], int nd = S) {
   double u = (e == 0) ? x * 128.0 : ldexp(x, 7 - e);
 #pragma unroll
   for (int p = 0; p < S; p++) {
+    if (p >= nd) {
+      d[p] = 0.0;
+      continue;
+    }
     const double a = rint(u);
     d[p] = a;
     u = (u - a) * RADIX;
@@ -441,7 +447,7 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
     double dB[2][S], dS[2][S];
 #pragma unroll
     for (int e = 0; e < 2; e++) {
-      split_value<S>(x[e] * sB / 128.0, 0, dB[e]);  // x * 2^(7-e) with e folded in sB
+      split_value<S>(x[e] * sB / 128.0, 0, dB[e], sz);  // x * 2^(7-e) with e folded in sB
       if (a.want_ds) {
         if (kc + e < a.kx0_end) {
 #pragma unroll
@@ -450,11 +456,14 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
 #pragma unroll
           for (int p = 0; p < S; p++) dS[e][p] = dB[e][p];
         } else {
-          split_value<S>(x[e] * sS / 128.0, 0, dS[e]);
+          split_value<S>(x[e] * sS / 128.0, 0, dS[e], sz);
         }
       }
 #pragma unroll
-      for (int p = 0; p < S; p++) lt[p][(2 * cp + e) * PITCH + ii] = (int8_t)(int)dB[e][p];
+      for (int p = 0; p < S; p++) {
+        if (p >= sz) break;
+        lt[p][(2 * cp + e) * PITCH + ii] = (int8_t)(int)dB[e][p];
+      }
     }
     if (a.want_ds) {
 #pragma unroll
@@ -834,7 +843,7 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
           for (int v = 0; v < 16; v++) {
             double t = 0.0;
 #pragma unroll
-            for (int l = NL - 1; l >= 0; l--) t += (double)acc[l][i][j][v] * rinv(l);
+            for (int l = NL - 1; l >= 0; l--) t = fma((double)acc[l][i][j][v], rinv(l), t);
             facc[i][j][v] += t;
           }
 #pragma unroll
@@ -958,9 +967,11 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
           for (int r = 0; r < 4; r++) pd[r] += q4[r] * rinv(p);
         }
       }
+      // explicit fma / __dmul_rn: the same rounding in every kernel that
+      // inlines this body (single and grouped launches) and every build
       auto value = [&](int r) {
-        return al * (dv[r] * ldexp(pd[r], ez - 7) +
-                     ldexp(level_sum(4 * g + r), fe[r] + ez - 14));
+        return __dmul_rn(al, fma(dv[r], ldexp(pd[r], ez - 7),
+                                 ldexp(level_sum(4 * g + r), fe[r] + ez - 14)));
       };
       double *o = a.out + (size_t)mg * a.ldo + nn;
       if (!edge) {
@@ -972,14 +983,14 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
 #pragma unroll
           for (int r = 0; r < 4; r++) old[r] = o[(size_t)r * a.ldo];
 #pragma unroll
-          for (int r = 0; r < 4; r++) o[(size_t)r * a.ldo] = value(r) + a.beta * old[r];
+          for (int r = 0; r < 4; r++) o[(size_t)r * a.ldo] = fma(a.beta, old[r], value(r));
         }
       } else {
 #pragma unroll
         for (int r = 0; r < 4; r++)
           if (mg + r < a.M && nn < a.N) {
             double *ov = o + (size_t)r * a.ldo;
-            *ov = (a.beta != 0.0) ? value(r) + a.beta * (*ov) : value(r);
+            *ov = (a.beta != 0.0) ? fma(a.beta, *ov, value(r)) : value(r);
           }
       }
     }
@@ -1119,8 +1130,8 @@ __device__ __forceinline__ void split_reduce_body(const oa_sreduce_args &a, int 
   double s = 0.0;
   for (int z = 0; z < a.nz; z++) s += a.part[((long)z * ntiles + tile) * (TILE * TILE) + e];
   double g = 0.0;
-  for (int k = 0; k < a.kx0_end; k++) g += a.psi[(long)i * a.ld + k] * a.psi[(long)j * a.ld + k];
-  s = a.alpha * s + 0.5 * a.alpha * g;
+  for (int k = 0; k < a.kx0_end; k++) g = fma(a.psi[(long)i * a.ld + k], a.psi[(long)j * a.ld + k], g);
+  s = fma(a.alpha, s, __dmul_rn(0.5 * a.alpha, g));
   a.S[(long)i * n + j] = s;
   a.S[(long)j * n + i] = s;
   if (a.maxdev) {
