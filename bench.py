@@ -87,11 +87,18 @@ def calibrate_copy():
 
 
 # HBM bytes per launch of the dominant kernel at n=1024, npw=262144, measured
-# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (separate runs) and scaled by
-# the same counters on a 1 GiB device copy (fetch x2.000, write x1.000).
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (separate runs of this
+# script's large workload) and scaled by the same counters on a 1 GiB device
+# copy (fetch x2.000, write x1.000); the profile each figure comes from is
+# reported beside it.  Counters cannot be read inside this process, so the
+# figure is the profiled one, not one measured in the reporting run.
 TRAFFIC = {
-    "fp64": 35.80e9 + 4.29e9,     # gemm_f64_kernel T*psi: read 35.80 GB, write 4.29 GB
-    "int8x5": 14.26e9 + 4.29e9,   # digit back-multiply: read 14.26 GB, write 4.29 GB
+    # gemm_f64_kernel T*psi: read 35.80 GB, write 4.29 GB
+    "fp64": (35.80e9 + 4.29e9, "profiles/ortho_step_n1024_npw262144_hbm_traffic.md"),
+    # digit back-multiply (one launch per tile): read 14.26 GB, write 4.29 GB
+    "int8x5": (14.26e9 + 4.29e9, "profiles/ortho_step_n1024_npw262144_hbm_traffic.md"),
+    # digit back-multiply (persistent): read 7.385 GB, write 4.304 GB
+    "int8x6a": (7.385e9 + 4.304e9, "profiles/ortho_step_int8x6a_n1024_npw262144_hbm_traffic.md"),
 }
 
 
@@ -766,7 +773,10 @@ def main():
             "peak": peak,
             "unit": unit,
             "frac": achieved / peak,
-            "traffic": TRAFFIC.get(head["prec"]) if args.workload == "large" else None,
+            "traffic": (TRAFFIC[head["prec"]][0]
+                        if args.workload == "large" and head["prec"] in TRAFFIC else None),
+            "traffic_source": (TRAFFIC[head["prec"]][1]
+                               if args.workload == "large" and head["prec"] in TRAFFIC else None),
             "algorithmic_per_launch": alg,
         },
     }
