@@ -29,12 +29,14 @@
 //
 // Kernels (each a body over work items, launched either for one instance
 // with arguments by value or grouped over many instances from device tables):
-//   chunk_max     max |psi| per chunk (two variants: without / with the
-//                 leading kx0 columns), order-free via atomicMax on the IEEE
-//                 bits of non-negative doubles.
+//   chunk_max     max |psi| per chunk (two variants: for the S layout, the
+//                 leading kx0 columns left out or at 1/sqrt(2); plain),
+//                 order-free via atomicMax on the IEEE bits of non-negative
+//                 doubles.
 //   slice_psi     psi -> digits in two layouts: DS[p][row][k] (rows
-//                 contiguous in k, for S = psi psi^T; kx0 columns zeroed,
-//                 their term is added in FP64) and DB[p][k][row] (rows
+//                 contiguous in k, for S = psi psi^T; kx0 columns zeroed
+//                 and their term added in FP64, or for a wide plane kept
+//                 times 1/sqrt(2)) and DB[p][k][row] (rows
 //                 contiguous in the state index, for T psi), the second
 //                 through an LDS transpose.
 //   slice_rows    an n x n matrix -> digits with one exponent per row.
@@ -46,7 +48,7 @@
 //                 TILE 128: 8 waves as 4 x 2 with 32 x 64 wave tiles, two
 //                 waves per SIMD;
 //                 TILE 64: 256 threads, 4 waves as 2 x 2, wave tile 32 x 32.
-//   split_reduce  fixed-order sum of the partial tiles, FP64 G=0 term,
+//   split_reduce  fixed-order sum of the partial tiles, FP64 kx0 term,
 //                 mirror, max|S - d I|.
 #include <math.h>
 
@@ -65,6 +67,13 @@ typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 constexpr int DBK = 32;   // chunk granularity (elements); a stage holds 32 bytes per row
 constexpr int SMAX = 6;
 constexpr int SLT = 64;   // slicing tile (rows x columns)
+// The leading kx0 columns (the kx = 0 plane, stored once for +G and -G)
+// count half in S.  Up to KX0_FP64_MAX of them are left out of the S-layout
+// digits and their term is added in FP64 by the reduce (exact, and cheap for
+// a few columns); a wider plane is kept in the digits times 1/sqrt(2), so the
+// digit products carry the weight 1/2 at no extra cost per column.
+constexpr double KX0W = 0.70710678118654752440;
+constexpr int KX0_FP64_MAX = 32;
 constexpr int ADA_SMIN = 2;      // fewest slices an adaptive chunk keeps
 constexpr double ADA_THETA = 254.0 / 4.0;  // budget of the dropped chunks, in units of B_S
 constexpr int ADA_SMIN_MAX = 4;  // adaptive mode needs at least this many slices at most
@@ -250,13 +259,15 @@ __global__ void fill_u64(unsigned long long *p, long n) {
 }
 
 // ------------------------------------------------------------ chunk max
-// Item (z, row group of 8): max over rows x chunk columns; maxS skips the
-// leading kx0 columns, maxB includes them.
+// Item (z, row group of 8): max over rows x chunk columns.  maxB is the
+// plain maximum; maxS skips the leading kx0 columns, or counts them at
+// |x| / sqrt(2) when the S-layout digits hold them (kx0_end > KX0_FP64_MAX).
 __device__ __forceinline__ void chunk_max_body(const oa_chunkmax_args &a, int item) {
   const int z = item % a.nz, r0 = (item / a.nz) * 8;
   const int k0 = z * a.kchunk, k1 = min(a.K, k0 + a.kchunk);
   double ms = 0.0, mb = 0.0;
   const bool vec = ((a.ld & 1) == 0) && (((uintptr_t)a.psi & 15) == 0) && ((k0 & 1) == 0);
+  const double w0 = (a.kx0_end > KX0_FP64_MAX) ? KX0W : 0.0;  // weight of a kx0 column in maxS
   for (int r = r0; r < min(a.n, r0 + 8); r++) {
     const double *row = a.psi + (long)r * a.ld;
     int k = k0 + 2 * (int)threadIdx.x;
@@ -271,8 +282,8 @@ __device__ __forceinline__ void chunk_max_body(const oa_chunkmax_args &a, int it
           const double x0 = fabs(v[u].x), x1 = fabs(v[u].y);
           const int kk = k + u * 512;
           mb = fmax(mb, fmax(x0, x1));
-          if (kk >= a.kx0_end) ms = fmax(ms, x0);
-          if (kk + 1 >= a.kx0_end) ms = fmax(ms, x1);
+          ms = fmax(ms, (kk >= a.kx0_end) ? x0 : x0 * w0);
+          ms = fmax(ms, (kk + 1 >= a.kx0_end) ? x1 : x1 * w0);
         }
       }
     }
@@ -280,11 +291,11 @@ __device__ __forceinline__ void chunk_max_body(const oa_chunkmax_args &a, int it
     for (; k < k1; k += 512) {
       const double x0 = fabs(row[k]);
       mb = fmax(mb, x0);
-      if (k >= a.kx0_end) ms = fmax(ms, x0);
+      ms = fmax(ms, (k >= a.kx0_end) ? x0 : x0 * w0);
       if (k + 1 < k1) {
         const double x1 = fabs(row[k + 1]);
         mb = fmax(mb, x1);
-        if (k + 1 >= a.kx0_end) ms = fmax(ms, x1);
+        ms = fmax(ms, (k + 1 >= a.kx0_end) ? x1 : x1 * w0);
       }
     }
   }
@@ -403,7 +414,8 @@ __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
 // Item = 64 rows x 64 columns (item % (Kp/64) selects the columns).  Thread t
 // owns columns 2(t%32), 2(t%32)+1 of rows t/32 + 8q (q < 8): coalesced
 // 16-byte reads.  DS[p] (rows np, contraction Kp) gets the S-layout digits
-// (kx0 columns zeroed, their term is exact FP64); DB[p] (rows Kp, contraction
+// (kx0 columns zeroed, or times 1/sqrt(2) for a wide plane); DB[p] (rows Kp,
+// contraction
 // np) gets all columns through an LDS transpose.  want_ds == 0 skips DS,
 // want_db == 0 skips DB.  (Staging DS through LDS as well, for 16-byte stores,
 // measured slower: the second buffer halves the resident workgroups.)
@@ -451,8 +463,12 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
       split_value<S>(x[e] * sB / 128.0, 0, dB[e], sz);  // x * 2^(7-e) with e folded in sB
       if (a.want_ds) {
         if (kc + e < a.kx0_end) {
+          if (a.kx0_end > KX0_FP64_MAX) {
+            split_value<S>(x[e] * KX0W * sS / 128.0, 0, dS[e], sz);
+          } else {
 #pragma unroll
-          for (int p = 0; p < S; p++) dS[e][p] = 0.0;
+            for (int p = 0; p < S; p++) dS[e][p] = 0.0;
+          }
         } else if (sS == sB) {
 #pragma unroll
           for (int p = 0; p < S; p++) dS[e][p] = dB[e][p];
@@ -1113,7 +1129,8 @@ __attribute__((amdgpu_waves_per_eu(DigitCfg<DT, S, MODE, NLB, TILE>::WPE,
 
 // ------------------------------------------------------------ reduce
 // Item = 256 consecutive elements of the upper-triangle partial tiles: fixed
-// order sum over the splits, the FP64 kx0 term (weight 1/2), mirror,
+// order sum over the splits, the FP64 kx0 term (weight 1/2) of a plane of
+// at most KX0_FP64_MAX columns (a wider one is in the digits), mirror,
 // max|S - d I|.
 template <int TILE>
 __device__ __forceinline__ void split_reduce_body(const oa_sreduce_args &a, int item) {
@@ -1133,7 +1150,9 @@ __device__ __forceinline__ void split_reduce_body(const oa_sreduce_args &a, int 
   double s = 0.0;
   for (int z = 0; z < a.nz; z++) s += a.part[((long)z * ntiles + tile) * (TILE * TILE) + e];
   double g = 0.0;
-  for (int k = 0; k < a.kx0_end; k++) g = fma(a.psi[(long)i * a.ld + k], a.psi[(long)j * a.ld + k], g);
+  if (a.kx0_end <= KX0_FP64_MAX)
+    for (int k = 0; k < a.kx0_end; k++)
+      g = fma(a.psi[(long)i * a.ld + k], a.psi[(long)j * a.ld + k], g);
   s = fma(a.alpha, s, __dmul_rn(0.5 * a.alpha, g));
   a.S[(long)i * n + j] = s;
   a.S[(long)j * n + i] = s;

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 
 import torch
 
@@ -57,6 +58,9 @@ class Handle:
         h = ctypes.c_void_p()
         check(L.oa_create(ctypes.byref(h), self.device), "oa_create")
         self.h = h
+        # batches made on this handle: destroyed before it (a batch's
+        # teardown synchronizes on the handle's device and stream)
+        self._batches = weakref.WeakSet()
         self.use_stream(stream)
 
     def use_stream(self, stream: torch.cuda.Stream | None = None) -> None:
@@ -72,6 +76,8 @@ class Handle:
 
     def close(self) -> None:
         if getattr(self, "h", None):
+            for b in list(getattr(self, "_batches", ())):
+                b.close()
             _lib.lib().oa_destroy(self.h)
             self.h = None
 
@@ -380,9 +386,12 @@ class Batch:
 
     def __init__(self, handle: Handle | None = None):
         self.handle = handle or default_handle()
+        if not getattr(self.handle, "h", None):
+            raise ValueError("the handle is closed")
         b = ctypes.c_void_p()
         check(_lib.lib().oa_batch_create(self.handle.h, ctypes.byref(b)), "oa_batch_create")
         self.b = b
+        self.handle._batches.add(self)
         self._keep = []
         self.count = 0
 

@@ -114,3 +114,22 @@ def test_batch_mixed_maxit_and_nonconvergence(handle):
     S, T, out, it = alone(handle, psis[2], "int8x5")
     assert torch.equal(b.outs[2], out)
     b.close()
+
+
+def test_handle_close_closes_its_batches():
+    """A batch made on a handle is destroyed with it: closing the handle
+    first (as a session teardown does) must leave nothing for the batch's
+    own close or finalizer to touch."""
+    from openatom_amd.ortho import Handle
+    h = Handle(0)
+    b = Batch(h)
+    psi = make_states(32, 512, eps=1e-3, seed=1, device=DEV, rng="torch")
+    b.plan([psi], nkx0=1, nspin=2, tol=1e-12, maxit=10, prec="int8x5")
+    b.launch()
+    torch.cuda.synchronize()
+    h.close()
+    assert b.b is None
+    b.close()
+    del b
+    with pytest.raises(ValueError):
+        Batch(h)
