@@ -403,6 +403,17 @@ __global__ __launch_bounds__(256) void slice_count_grouped(const oa_count_args *
   });
 }
 
+// Calls f(integral_constant<int, V>) with V = v clamped into [LO, HI].
+template <int LO, int HI, class F>
+__device__ __forceinline__ void level_switch(int v, F &&f) {
+  if constexpr (LO < HI) {
+    if (v <= LO) f(std::integral_constant<int, LO>{});
+    else level_switch<LO + 1, HI>(v, f);
+  } else {
+    f(std::integral_constant<int, HI>{});
+  }
+}
+
 // ------------------------------------------------------------ slicing
 // k-blocked element offset (elements) of (row r, contraction c) in one slice.
 template <int DT>
@@ -419,13 +430,18 @@ __host__ __device__ __forceinline__ size_t kb_off(int r, int c, size_t rows) {
 // np) gets all columns through an LDS transpose.  want_ds == 0 skips DS,
 // want_db == 0 skips DB.  (Staging DS through LDS as well, for 16-byte stores,
 // measured slower: the second buffer halves the resident workgroups.)
-template <int DT, int S>
-__device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
+constexpr int SLPITCH = SLT + 16;  // LDS row pitch in bytes (16-byte aligned rows)
+
+// SZ: digits kept for the item's chunk (the item's 64 columns lie in one
+// chunk), a compile-time count so that the digit loops carry no per-digit
+// branch.
+template <int DT, int S, int SZ>
+__device__ __forceinline__ void slice_psi_sz(const oa_slice_args &a, int item,
+                                             int8_t (*lt)[SLT * SLPITCH]) {
   typedef typename DTraits<DT>::elem E;
   constexpr int ES = DTraits<DT>::ES;
-  constexpr int PITCH = SLT + 16;  // LDS row pitch in bytes (16-byte aligned rows)
-  // digits are integers |a| <= 127 in every type: the transpose holds int8
-  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * PITCH];
+  constexpr int PITCH = SLPITCH;
+  constexpr int sz = SZ;
   E *DS = reinterpret_cast<E *>(a.DS);
   E *DB = reinterpret_cast<E *>(a.DB);
   const int nkt = a.Kp / SLT;
@@ -435,8 +451,6 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
   const int kc = k0 + 2 * cp;  // first of this thread's two columns
   const int z = min(kc, a.K - 1 > 0 ? a.K - 1 : 0) / a.kchunk;
   const double sS = chunk_scale(a.maxS, z), sB = chunk_scale(a.maxB, z);
-  // digits kept for this chunk (the item's 64 columns lie in one chunk)
-  const int sz = a.slices ? a.slices[z] : S;
   const size_t sds = (size_t)a.np * a.Kp, sdb = (size_t)a.Kp * a.np;
   // all eight 16-byte loads first (eight in flight per thread)
   double xq[8][2];
@@ -525,6 +539,17 @@ __device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item)
     }
   }
   __syncthreads();
+}
+
+template <int DT, int S>
+__device__ __forceinline__ void slice_psi_body(const oa_slice_args &a, int item) {
+  // digits are integers |a| <= 127 in every type: the transpose holds int8
+  __shared__ __attribute__((aligned(16))) int8_t lt[S][SLT * SLPITCH];
+  const int nkt = a.Kp / SLT;
+  const int k0 = (item % nkt) * SLT;
+  const int z = min(k0, a.K - 1 > 0 ? a.K - 1 : 0) / a.kchunk;
+  const int sz = a.slices ? a.slices[z] : S;
+  level_switch<1, S>(sz, [&](auto c) { slice_psi_sz<DT, S, decltype(c)::value>(a, item, lt); });
 }
 
 template <int DT, int S>
@@ -1016,16 +1041,6 @@ __device__ __forceinline__ void digit_core(const oa_digit_args &a, int bi, int b
   ring_barrier();
 }
 
-// Calls f(integral_constant<int, V>) with V = v clamped into [LO, HI].
-template <int LO, int HI, class F>
-__device__ __forceinline__ void level_switch(int v, F &&f) {
-  if constexpr (LO < HI) {
-    if (v <= LO) f(std::integral_constant<int, LO>{});
-    else level_switch<LO + 1, HI>(v, f);
-  } else {
-    f(std::integral_constant<int, HI>{});
-  }
-}
 
 // Item decoding.  An adaptive split spreads the chunks over the XCDs: the
 // dispatcher gives XCD x the contiguous item range x*per .. (x+1)*per - 1,
